@@ -1,0 +1,93 @@
+// Host side of the C ABI: the device handle, resident buffers, the error
+// slot and the staging helpers every entry point shares.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/cft.h"
+
+// The library's own limits, published through cft_get_caps.
+namespace cftl {
+constexpr uint32_t kMaxDeposits = 1u << 20;     // deposit slots a lane
+constexpr uint32_t kMaxInsns = 1u << 20;        // instructions an image holds
+constexpr uint32_t kMaxConsts = 512;            // constants an insn addresses
+constexpr uint32_t kDefaultDepth = 256;         // scratch slots a lane
+constexpr uint32_t kMaxDepth = 2048;
+constexpr uint32_t kSeqFeatures =
+    CFT_SEQ_FEAT_WIDE_CONST | CFT_SEQ_FEAT_REGS32 | CFT_SEQ_FEAT_BANK_PTR | CFT_SEQ_FEAT_KX9 |
+    CFT_ALU_EXT_IMUL | CFT_SEQ_FEAT_SCRATCH | CFT_SEQ_FEAT_SCRATCH_IO |
+    CFT_SEQ_FEAT_SCRATCH_STRICT | CFT_SEQ_FEAT_SCALAR | CFT_SEQ_FEAT_LANE_MASK |
+    CFT_SEQ_FEAT_SCRATCH_STEP | CFT_SEQ_FEAT_LANE_FLAGS | CFT_SEQ_FEAT_FLAG_CONTROL;
+}
+
+// One growable device allocation.
+struct DevArena {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct cft_buffer {
+    cft_device *dev = nullptr;
+    void *host = nullptr;          // the mirror the caller holds (pinned)
+    size_t bytes = 0;
+    void *devp = nullptr;          // the device copy
+    bool host_valid = true;        // mirror holds the current bytes
+    bool dev_valid = false;        // device copy holds the current bytes
+    uint64_t resident_binds = 0, staged_binds = 0;
+    char why[112] = {0};
+};
+
+struct cft_device {
+    int hip_index = 0;
+    hipStream_t stream = nullptr;
+    uint32_t sticky = 0;           // 754 7.1 status word
+    bool poisoned = false;
+    uint32_t scratch_depth = cftl::kDefaultDepth;
+    int num_cus = 256;
+    DevArena stage[8];             // per operand role
+    uint32_t *dwords = nullptr;    // device: [0] flags, [1] status
+    uint32_t *hwords = nullptr;    // pinned mirror of dwords
+    std::vector<cft_buffer *> bufs;
+};
+
+// ---- the error slot (process-global, as the reference's) ----------------
+void cft_clear_error();
+void cft_set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+// Emit a call's flags: OR into the device's word, write flags_out.
+void cft_emit_flags(cft_device *dev, uint32_t acc, uint32_t *flags_out);
+
+// ---- device plumbing -------------------------------------------------------
+
+// Make dev's GPU current.  CFT_ERR_INTERNAL (with a sentence) on a poisoned
+// handle.
+cft_status dev_enter(cft_device *dev, const char *who);
+
+// A HIP failure after work may have started: poison the handle.
+cft_status dev_fail(cft_device *dev, hipError_t e, const char *who, const char *what);
+
+// Grow arena to at least bytes.  CFT_ERR_OUT_OF_MEMORY before anything ran.
+cft_status arena_reserve(cft_device *dev, DevArena &a, size_t bytes, const char *who);
+
+// Operand binding.  An input window [p, p+bytes) that lies inside a resident
+// buffer is served from its device copy (refreshed first if stale); any other
+// window is copied into the role's staging arena.  *dptr receives the device
+// address.
+cft_status bind_in(cft_device *dev, int role, const void *p, size_t bytes, const void **dptr,
+                   const char *who);
+// Output window: a buffer's device copy (made current first), or the role's
+// arena; *buf receives the buffer or NULL (then the caller copies back).
+cft_status bind_out(cft_device *dev, int role, void *p, size_t bytes, void **dptr,
+                    cft_buffer **buf, const char *who);
+
+// The buffer containing [p, p+bytes), or NULL.
+cft_buffer *find_buffer(cft_device *dev, const void *p, size_t bytes);
+
+// Kernel grid for n lanes of one element each.
+int grid_for(const cft_device *dev, size_t n);

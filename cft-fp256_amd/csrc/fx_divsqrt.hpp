@@ -1,0 +1,329 @@
+// Correctly-rounded division and square root on fixed limbs.
+//
+// The contract (python/cft_golden/softfloat.py:440-506, read as text; the
+// reference library reaches the same bits through a Newton/Markstein
+// composition, host/src/divsqrt.c): specials by operand class, otherwise the
+// exact quotient / root carried past the rounding position with the
+// remainder's nonzeroness as the sticky bit, then the one round_pack.  Any
+// method that produces the exact floor and an exact "remainder != 0" gives
+// the same bits and flags; the method here is ours:
+//
+//   division   schoolbook long division in radix 2^32 (Knuth's algorithm D):
+//              each quotient digit estimated from the top limbs in binary64,
+//              then an exact multiply-subtract and add-back, so the estimate
+//              only decides how many corrections run.
+//   sqrt       the radix-2^32 digit recurrence: bring down two limbs, choose
+//              the largest digit d with (2rB + d) d <= R, exactly, starting
+//              one above a binary64 estimate and stepping down.
+#pragma once
+
+#include "fx_arith.hpp"
+
+namespace fx {
+
+// Knuth D: Q = floor(X / D), X = U (2*NL limbs, U[2NL] = 0 appended), D
+// normalised (top bit of D[NL-1] set), quotient NL digits.  Returns Q and
+// whether the remainder is nonzero.
+template <int NL>
+__device__ __forceinline__ void long_div(uint32_t (&U)[2 * NL + 1], const uint32_t (&D)[NL],
+                                         uint32_t (&Q)[NL], bool &rem_nz)
+{
+    double dtop = (double)D[NL - 1] * 4294967296.0;
+    if constexpr (NL >= 2)
+        dtop += (double)D[NL - 2];
+    const double rinv = 1.0 / dtop;
+#pragma unroll
+    for (int j = NL - 1; j >= 0; j--) {
+        // estimate from U[j+NL], U[j+NL-1], U[j+NL-2] against D's top two
+        uint32_t qhat;
+        if (U[j + NL] >= D[NL - 1]) {
+            qhat = 0xFFFFFFFFu;
+        } else {
+            const double num = (double)U[j + NL] * 18446744073709551616.0 +
+                               (double)U[j + NL - 1] * 4294967296.0 +
+                               (double)(j + NL - 2 >= 0 ? U[(j + NL - 2) < 0 ? 0 : j + NL - 2] : 0u);
+            double qd = num * rinv + 1.0;        // at or above the digit
+            qd = qd > 4294967295.0 ? 4294967295.0 : (qd < 0.0 ? 0.0 : qd);
+            qhat = (uint32_t)qd;
+        }
+        // U[j .. j+NL] -= qhat * D
+        uint32_t carry = 0, br = 0;
+#pragma unroll
+        for (int i = 0; i < NL; i++) {
+            const uint64_t p = (uint64_t)qhat * D[i] + carry;
+            carry = (uint32_t)(p >> 32);
+            const uint64_t t = (uint64_t)U[j + i] - (uint32_t)p - br;
+            U[j + i] = (uint32_t)t;
+            br = (uint32_t)(t >> 63);
+        }
+        {
+            const uint64_t t = (uint64_t)U[j + NL] - carry - br;
+            U[j + NL] = (uint32_t)t;
+            br = (uint32_t)(t >> 63);
+        }
+        // negative: add D back until it is not (the exact test decides)
+        bool neg = br != 0;
+        while (neg) {
+            uint64_t c = 0;
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                c += (uint64_t)U[j + i] + D[i];
+                U[j + i] = (uint32_t)c;
+                c >>= 32;
+            }
+            c += U[j + NL];
+            U[j + NL] = (uint32_t)c;
+            neg = (c >> 32) == 0;                // no carry out: still negative
+            qhat--;
+        }
+        Q[j] = qhat;
+    }
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        o |= U[i];
+    rem_nz = o != 0;
+}
+
+template <int NL>
+__device__ __forceinline__ void div_op(const uint32_t (&xa)[NL], const uint32_t (&xb)[NL], int rnd,
+                                       uint32_t (&out)[NL], uint32_t &flags)
+{
+    using Fm = F<NL>;
+    constexpr int P = Fm::P;
+    Unp<NL> ua, ub;
+    unpack<NL>(xa, ua);
+    unpack<NL>(xb, ub);
+    const int sq = ua.sign ^ ub.sign;
+    if (ua.nan || ub.nan) {
+        enc_qnan<NL>(out);
+        flags |= (ua.snan || ub.snan) ? FL_INV : 0u;
+        return;
+    }
+    if (ua.inf) {
+        if (ub.inf) { enc_qnan<NL>(out); flags |= FL_INV; }
+        else enc_inf<NL>(out, sq);
+        return;
+    }
+    if (ub.inf) { enc_zero<NL>(out, sq); return; }
+    if (ub.zero) {
+        if (ua.zero) { enc_qnan<NL>(out); flags |= FL_INV; }
+        else { enc_inf<NL>(out, sq); flags |= FL_DZ; }
+        return;
+    }
+    if (ua.zero) { enc_zero<NL>(out, sq); return; }
+
+    // Q = floor(ma * 2^k / mb) with exactly P+3 bits: k = P+2 when ma >= mb
+    bool ge = true;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        ge = ua.m[i] != ub.m[i] ? ua.m[i] > ub.m[i] : ge;
+    const int k = ge ? P + 2 : P + 3;
+    constexpr int SD = 32 * NL - P;              // divisor normalisation
+    uint32_t D[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        D[i] = ub.m[i];
+    shl<NL>(D, SD);
+    // U = ma << (k + SD) = (ma << (k + SD - 32NL)) in the top NL limbs
+    uint32_t U[2 * NL + 1];
+    uint32_t hi[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        hi[i] = ua.m[i];
+    shl<NL>(hi, (uint32_t)(k + SD - 32 * NL));   // 2 or 3
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        U[i] = 0;
+        U[NL + i] = hi[i];
+    }
+    U[2 * NL] = 0;
+    uint32_t Qd[NL];
+    bool rem_nz;
+    long_div<NL>(U, D, Qd, rem_nz);
+    // Q has P+3 bits: K = Q >> 3, guard bit 2, sticky bits 1..0 and the rest
+    uint32_t K[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        K[i] = alignbit(i + 1 < NL ? Qd[i + 1] : 0u, Qd[i], 3);
+    const bool g = (Qd[0] >> 2) & 1u;
+    const bool s = (Qd[0] & 3u) != 0 || rem_nz;
+    round_pack_norm<NL>(sq, K, g, s, ua.e - ub.e - k + P + 2, rnd, out, flags);
+}
+
+// r = floor(sqrt(M)), M 2*NL limbs with its top bit at 64NL-1 or 64NL-2;
+// r has NL limbs with its top bit set.  rem_nz = (r*r != M).
+template <int NL>
+__device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_t (&r)[NL],
+                                            bool &rem_nz)
+{
+    // R: the running remainder, at most NL + 2 limbs; r grows one digit a step
+    uint32_t R[NL + 2];
+#pragma unroll
+    for (int i = 0; i < NL + 2; i++)
+        R[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        r[i] = 0;
+    // first digit: isqrt of the top 64 bits
+    {
+        const uint64_t g = ((uint64_t)M[2 * NL - 1] << 32) | M[2 * NL - 2];
+        uint64_t d = (uint64_t)sqrt((double)g);
+        if (d > 0xFFFFFFFFull) d = 0xFFFFFFFFull;
+        while (d * d > g) d--;
+        while (d < 0xFFFFFFFFull && (d + 1) * (d + 1) <= g) d++;
+        const uint64_t rem = g - d * d;
+        R[0] = (uint32_t)rem;
+        R[1] = (uint32_t)(rem >> 32);
+        r[NL - 1] = (uint32_t)d;                 // r kept left-aligned: digits fill down
+    }
+#pragma unroll
+    for (int k = 1; k < NL; k++) {
+        // r so far: digits r[NL-k .. NL-1] (k limbs); R < 2^33 * 2^(32(k-1))
+        // bring down two limbs: R = R * B^2 + (M[2NL-2k-1] : M[2NL-2k-2])
+#pragma unroll
+        for (int i = NL + 1; i >= 2; i--)
+            R[i] = R[i - 2];
+        R[1] = M[2 * NL - 2 * k - 1];
+        R[0] = M[2 * NL - 2 * k - 2];
+        // V = 2r as k+1 limbs (low first)
+        uint32_t V[NL + 1];
+#pragma unroll
+        for (int i = 0; i <= NL; i++)
+            V[i] = 0;
+#pragma unroll
+        for (int i = 0; i < k; i++) {
+            const uint32_t lo = i > 0 ? r[NL - k + i - 1] : 0u;
+            V[i] = (r[NL - k + i] << 1) | (lo >> 31);
+        }
+        V[k] = r[NL - 1] >> 31;
+        // estimate d = R / (V * B), from the top limbs as doubles
+        double Rd = 0.0, Vd = 0.0;
+#pragma unroll
+        for (int i = 0; i < NL + 2; i++)
+            if (i <= k + 2 && i >= k - 1)
+                Rd += ldexp((double)R[i], 32 * i);
+#pragma unroll
+        for (int i = 0; i <= NL; i++)
+            if (i <= k && i >= k - 2)
+                Vd += ldexp((double)V[i], 32 * (i + 1));
+        double dd = Rd / Vd + 2.0;               // at or above the digit
+        dd = dd > 4294967295.0 ? 4294967295.0 : (dd < 0.0 ? 0.0 : dd);
+        uint32_t d = (uint32_t)dd;
+        // T = (V * B + d) * d, k + 3 limbs
+        uint32_t T[NL + 2];
+#pragma unroll
+        for (int i = 0; i < NL + 2; i++)
+            T[i] = 0;
+        {
+            // (V*B + d) has limbs [d, V0, V1, .., Vk]
+            uint32_t carry = 0;
+#pragma unroll
+            for (int i = 0; i <= k + 1; i++) {
+                const uint32_t w = i == 0 ? d : V[i - 1];
+                const uint64_t p = (uint64_t)w * d + carry;
+                T[i] = (uint32_t)p;
+                carry = (uint32_t)(p >> 32);
+            }
+            if (k + 2 < NL + 2)
+                T[k + 2] = carry;
+        }
+        // while T > R: T -= (V*B + 2d - 1); d--
+        auto t_gt_r = [&]() {
+            int c = 0;
+#pragma unroll
+            for (int i = 0; i < NL + 2; i++)
+                c = T[i] != R[i] ? (T[i] > R[i] ? 1 : -1) : c;
+            return c > 0;
+        };
+        while (t_gt_r()) {
+            // S = V*B + 2d - 1 as limbs (d >= 1 here, since T > R >= 0)
+            uint32_t Sv[NL + 2];
+            {
+                const uint64_t low = 2ull * d - 1ull;
+                Sv[0] = (uint32_t)low;
+                uint64_t c = low >> 32;
+#pragma unroll
+                for (int i = 1; i < NL + 2; i++) {
+                    c += (i - 1 <= k) ? V[i - 1] : 0u;
+                    Sv[i] = (uint32_t)c;
+                    c >>= 32;
+                }
+            }
+            uint32_t br = 0;
+#pragma unroll
+            for (int i = 0; i < NL + 2; i++) {
+                const uint64_t t = (uint64_t)T[i] - Sv[i] - br;
+                T[i] = (uint32_t)t;
+                br = (uint32_t)(t >> 63);
+            }
+            d--;
+        }
+        // R -= T
+        {
+            uint32_t br = 0;
+#pragma unroll
+            for (int i = 0; i < NL + 2; i++) {
+                const uint64_t t = (uint64_t)R[i] - T[i] - br;
+                R[i] = (uint32_t)t;
+                br = (uint32_t)(t >> 63);
+            }
+        }
+        r[NL - 1 - k] = d;
+    }
+    // r is left-aligned in r[0..NL) (all NL digits placed)
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < NL + 2; i++)
+        o |= R[i];
+    rem_nz = o != 0;
+}
+
+template <int NL>
+__device__ __forceinline__ void sqrt_op(const uint32_t (&xa)[NL], int rnd, uint32_t (&out)[NL],
+                                        uint32_t &flags)
+{
+    using Fm = F<NL>;
+    constexpr int P = Fm::P;
+    Unp<NL> ua;
+    unpack<NL>(xa, ua);
+    if (ua.nan) {
+        enc_qnan<NL>(out);
+        flags |= ua.snan ? FL_INV : 0u;
+        return;
+    }
+    if (ua.zero || (ua.inf && !ua.sign)) {       // +-0 and +inf are themselves
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+            out[i] = xa[i];
+        return;
+    }
+    if (ua.sign) {                               // negative, -inf included
+        enc_qnan<NL>(out);
+        flags |= FL_INV;
+        return;
+    }
+    // M = m << t, top bit at 64NL-1 or 64NL-2, (e - t) even
+    int t = 64 * NL - P;                         // top bit at 64NL - 1
+    if ((ua.e - t) & 1)
+        t--;
+    uint32_t M[2 * NL];
+#pragma unroll
+    for (int i = 0; i < 2 * NL; i++)
+        M[i] = i < NL ? ua.m[i] : 0u;
+    shl<2 * NL>(M, (uint32_t)t);
+    uint32_t r[NL];
+    bool rem_nz;
+    isqrt_limbs<NL>(M, r, rem_nz);
+    // r in [2^(32NL-1), 2^(32NL)): K = top P bits, then guard and sticky
+    constexpr int DROP = 32 * NL - P;            // = EXPW
+    uint32_t K[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        K[i] = alignbit(i + 1 < NL ? r[i + 1] : 0u, r[i], DROP);
+    const bool g = (r[0] >> (DROP - 1)) & 1u;
+    const bool s = (r[0] & ((1u << (DROP - 1)) - 1u)) != 0 || rem_nz;
+    round_pack_norm<NL>(0, K, g, s, 32 * NL - 1 + (ua.e - t) / 2, rnd, out, flags);
+}
+
+} // namespace fx

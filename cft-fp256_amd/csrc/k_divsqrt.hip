@@ -1,0 +1,80 @@
+// Correctly-rounded division and square root kernels, binary32 .. binary256,
+// one lane per element.  Replace the reference's composed routes (host/src/
+// divsqrt.c: ~25-30 elementwise passes, or a Newton program) with one pass:
+// the contract is "correctly rounded", so the bits and flags are the same.
+#include "fx_divsqrt.hpp"
+#include "kernels.h"
+
+using namespace fx;
+
+namespace {
+
+template <int NL>
+__device__ __forceinline__ void ld(const uint32_t *__restrict__ p, size_t i, uint32_t (&x)[NL])
+{
+#pragma unroll
+    for (int k = 0; k < NL; k++)
+        x[k] = p[i * NL + k];
+}
+
+template <int NL, int WHICH>
+__global__ __launch_bounds__(256) void k_divsqrt(const uint32_t *__restrict__ a,
+                                                 const uint32_t *__restrict__ b,
+                                                 uint32_t *__restrict__ d, size_t n, int rnd,
+                                                 uint32_t *__restrict__ flags_out)
+{
+    uint32_t fl = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t xa[NL], xb[NL], o[NL];
+        ld<NL>(a, i, xa);
+        if (WHICH == 0) {
+            ld<NL>(b, i, xb);
+            div_op<NL>(xa, xb, rnd, o, fl);
+        } else {
+            sqrt_op<NL>(xa, rnd, o, fl);
+        }
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+            d[i * NL + k] = o[k];
+    }
+    __shared__ uint32_t s_fl;
+    if (threadIdx.x == 0)
+        s_fl = 0;
+    __syncthreads();
+    if (fl)
+        atomicOr(&s_fl, fl);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fl)
+        atomicOr(flags_out, s_fl);
+}
+
+template <int NL>
+hipError_t launch(int which, int rnd, const void *a, const void *b, void *d, size_t n,
+                  uint32_t *flags, int grid, hipStream_t st)
+{
+    if (which == 0)
+        hipLaunchKernelGGL((k_divsqrt<NL, 0>), dim3(grid), dim3(256), 0, st,
+                           (const uint32_t *)a, (const uint32_t *)b, (uint32_t *)d, n, rnd,
+                           flags);
+    else
+        hipLaunchKernelGGL((k_divsqrt<NL, 1>), dim3(grid), dim3(256), 0, st,
+                           (const uint32_t *)a, (const uint32_t *)b, (uint32_t *)d, n, rnd,
+                           flags);
+    return hipGetLastError();
+}
+
+} // namespace
+
+extern "C" hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *b,
+                                   void *d, size_t n, uint32_t smask, uint32_t *flags, int grid,
+                                   hipStream_t st)
+{
+    (void)smask;
+    switch (fmt) {
+    case 0: return launch<1>(which, rnd, a, b, d, n, flags, grid, st);
+    case 1: return launch<2>(which, rnd, a, b, d, n, flags, grid, st);
+    case 2: return launch<4>(which, rnd, a, b, d, n, flags, grid, st);
+    default: return launch<8>(which, rnd, a, b, d, n, flags, grid, st);
+    }
+}

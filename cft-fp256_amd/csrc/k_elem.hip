@@ -1,0 +1,196 @@
+// Elementwise kernels: d[i] = op(a[i], b[i], c[i]) for the arithmetic group
+// (FMA, ADD, SUB, MUL) and the sign/compare/integer groups, binary32 ..
+// binary256.  The caller's arrays are dense little-endian elements (host/
+// include/cft.h:1166-1209); each lane owns one element at a time, loads it
+// with 16-byte vector loads and keeps it in VGPRs as 32-bit limbs.
+//
+// Replaces the reference's k_run / k_run256 / k_run128 (host/src/
+// backend_hip.cpp:484-665).  Launch shape: a grid-stride loop over a fixed
+// grid sized from the CU count, so the per-call flag word is ORed from one
+// atomic per workgroup rather than one per element or wave.
+#include "fx_arith.hpp"
+#include "fx_misc.hpp"
+#include "kernels.h"
+
+using namespace fx;
+
+namespace {
+
+template <int NL>
+__device__ __forceinline__ void load_el(const uint32_t *__restrict__ p, size_t i,
+                                        uint32_t (&x)[NL])
+{
+    if constexpr (NL == 8) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(p + i * 8);
+        const uint4 v0 = q[0], v1 = q[1];
+        x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
+        x[4] = v1.x; x[5] = v1.y; x[6] = v1.z; x[7] = v1.w;
+    } else if constexpr (NL == 4) {
+        const uint4 v0 = reinterpret_cast<const uint4 *>(p)[i];
+        x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
+    } else if constexpr (NL == 2) {
+        const uint2 v0 = reinterpret_cast<const uint2 *>(p)[i];
+        x[0] = v0.x; x[1] = v0.y;
+    } else {
+        x[0] = p[i];
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void store_el(uint32_t *__restrict__ p, size_t i,
+                                         const uint32_t (&x)[NL])
+{
+    if constexpr (NL == 8) {
+        uint4 *q = reinterpret_cast<uint4 *>(p + i * 8);
+        q[0] = make_uint4(x[0], x[1], x[2], x[3]);
+        q[1] = make_uint4(x[4], x[5], x[6], x[7]);
+    } else if constexpr (NL == 4) {
+        reinterpret_cast<uint4 *>(p)[i] = make_uint4(x[0], x[1], x[2], x[3]);
+    } else if constexpr (NL == 2) {
+        reinterpret_cast<uint2 *>(p)[i] = make_uint2(x[0], x[1]);
+    } else {
+        p[i] = x[0];
+    }
+}
+
+// One workgroup's flags into the call's word: wave OR by LDS atomics, then a
+// single global atomic.
+__device__ __forceinline__ void flush_flags(uint32_t fl, uint32_t *flags_out)
+{
+    __shared__ uint32_t s_fl;
+    if (threadIdx.x == 0)
+        s_fl = 0;
+    __syncthreads();
+    if (fl)
+        atomicOr(&s_fl, fl);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fl)
+        atomicOr(flags_out, s_fl);
+}
+
+// Arithmetic group.  OP is FMA/ADD/SUB/MUL; operands NULL when not read.
+template <int NL, int OP>
+__global__ __launch_bounds__(256) void k_arith(const uint32_t *__restrict__ a,
+                                               const uint32_t *__restrict__ b,
+                                               const uint32_t *__restrict__ c,
+                                               uint32_t *__restrict__ d, size_t n,
+                                               uint32_t smask, int rnd,
+                                               uint32_t *__restrict__ flags_out)
+{
+    uint32_t fl = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t xa[NL], xb[NL], xc[NL], o[NL];
+        load_el<NL>(a, (smask & 1u) ? 0 : i, xa);
+        if (OP == OP_FMA || OP == OP_MUL)
+            load_el<NL>(b, (smask & 2u) ? 0 : i, xb);
+        else
+#pragma unroll
+            for (int k = 0; k < NL; k++) xb[k] = 0;
+        if (OP != OP_MUL)
+            load_el<NL>(c, (smask & 4u) ? 0 : i, xc);
+        else
+#pragma unroll
+            for (int k = 0; k < NL; k++) xc[k] = 0;
+        fma_op<NL, OP>(xa, xb, xc, rnd, o, fl);
+        store_el<NL>(d, i, o);
+    }
+    flush_flags(fl, flags_out);
+}
+
+// Every other elementwise opcode (sign, min/max, compare, select, integer,
+// the seeds, and the unassigned ones): short, so one kernel switches on the
+// uniform opcode.  NULL operands read as +0.
+template <int NL>
+__global__ __launch_bounds__(256) void k_misc(const uint32_t *__restrict__ a,
+                                              const uint32_t *__restrict__ b,
+                                              const uint32_t *__restrict__ c,
+                                              uint32_t *__restrict__ d, size_t n,
+                                              uint32_t smask, int op,
+                                              uint32_t *__restrict__ flags_out)
+{
+    uint32_t fl = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t xa[NL], xb[NL], xc[NL], o[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) xa[k] = xb[k] = xc[k] = 0;
+        if (a) load_el<NL>(a, (smask & 1u) ? 0 : i, xa);
+        if (b) load_el<NL>(b, (smask & 2u) ? 0 : i, xb);
+        if (c) load_el<NL>(c, (smask & 4u) ? 0 : i, xc);
+        misc_op<NL>(op, xa, xb, xc, o, fl);
+        store_el<NL>(d, i, o);
+    }
+    flush_flags(fl, flags_out);
+}
+
+__global__ void k_fill32(uint32_t *p, uint32_t v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <int NL>
+hipError_t launch_arith(int op, const void *a, const void *b, const void *c, void *d, size_t n,
+                        uint32_t smask, int rnd, uint32_t *flags, int grid, hipStream_t st)
+{
+    const uint32_t *pa = (const uint32_t *)a, *pb = (const uint32_t *)b,
+                   *pc = (const uint32_t *)c;
+    uint32_t *pd = (uint32_t *)d;
+    switch (op) {
+    case OP_FMA:
+        hipLaunchKernelGGL((k_arith<NL, OP_FMA>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
+                           smask, rnd, flags);
+        break;
+    case OP_ADD:
+        hipLaunchKernelGGL((k_arith<NL, OP_ADD>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
+                           smask, rnd, flags);
+        break;
+    case OP_SUB:
+        hipLaunchKernelGGL((k_arith<NL, OP_SUB>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
+                           smask, rnd, flags);
+        break;
+    default:
+        hipLaunchKernelGGL((k_arith<NL, OP_MUL>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
+                           smask, rnd, flags);
+        break;
+    }
+    return hipGetLastError();
+}
+
+template <int NL>
+hipError_t launch_misc(int op, const void *a, const void *b, const void *c, void *d, size_t n,
+                       uint32_t smask, uint32_t *flags, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_misc<NL>), dim3(grid), dim3(256), 0, st, (const uint32_t *)a,
+                       (const uint32_t *)b, (const uint32_t *)c, (uint32_t *)d, n, smask, op,
+                       flags);
+    return hipGetLastError();
+}
+
+} // namespace
+
+extern "C" hipError_t cftk_elem(int fmt, int op, int rnd, const void *a, const void *b,
+                                const void *c, void *d, size_t n, uint32_t smask,
+                                uint32_t *flags, int grid, hipStream_t st)
+{
+    const bool arith = op >= OP_FMA && op <= OP_MUL;
+    switch (fmt) {
+    case 0: return arith ? launch_arith<1>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
+                         : launch_misc<1>(op, a, b, c, d, n, smask, flags, grid, st);
+    case 1: return arith ? launch_arith<2>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
+                         : launch_misc<2>(op, a, b, c, d, n, smask, flags, grid, st);
+    case 2: return arith ? launch_arith<4>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
+                         : launch_misc<4>(op, a, b, c, d, n, smask, flags, grid, st);
+    default: return arith ? launch_arith<8>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
+                          : launch_misc<8>(op, a, b, c, d, n, smask, flags, grid, st);
+    }
+}
+
+extern "C" hipError_t cftk_fill32(uint32_t *p, uint32_t v, size_t n, hipStream_t st)
+{
+    const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+    hipLaunchKernelGGL(k_fill32, dim3(grid > 0 ? grid : 1), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}

@@ -1,0 +1,63 @@
+// CPU build of the device limb arithmetic (cft-fp256_amd/csrc/fx_*.hpp) with
+// the portable fx_hw.hpp beside this file: one element at a time, for the
+// not-gpu tests to diff against the oracle.  Test infrastructure only.
+#include "fx_arith.hpp"
+#include "fx_divsqrt.hpp"
+#include "fx_misc.hpp"
+
+#include <string.h>
+
+using namespace fx;
+
+template <int NL>
+static void one(int op, int rnd, const uint32_t *a, const uint32_t *b, const uint32_t *c,
+                uint32_t *out, uint32_t *flags)
+{
+    uint32_t xa[NL], xb[NL], xc[NL], o[NL];
+    uint32_t fl = 0;
+    for (int i = 0; i < NL; i++) {
+        xa[i] = a ? a[i] : 0u;
+        xb[i] = b ? b[i] : 0u;
+        xc[i] = c ? c[i] : 0u;
+    }
+    switch (op) {
+    case 0: fma_op<NL, OP_FMA>(xa, xb, xc, rnd, o, fl); break;
+    case 1: fma_op<NL, OP_ADD>(xa, xb, xc, rnd, o, fl); break;
+    case 2: fma_op<NL, OP_SUB>(xa, xb, xc, rnd, o, fl); break;
+    case 3: fma_op<NL, OP_MUL>(xa, xb, xc, rnd, o, fl); break;
+    case 256: div_op<NL>(xa, xb, rnd, o, fl); break;
+    case 257: sqrt_op<NL>(xa, rnd, o, fl); break;
+    default: misc_op<NL>(op, xa, xb, xc, o, fl); break;
+    }
+    memcpy(out, o, sizeof o);
+    *flags = fl;
+}
+
+extern "C" int emu_run(int fmt, int op, int rnd, const void *a, const void *b, const void *c,
+                       void *d, size_t n, uint32_t *flags_out)
+{
+    const int nl = 1 << fmt;
+    uint32_t acc = 0;
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t *pa = a ? (const uint32_t *)a + i * nl : nullptr;
+        const uint32_t *pb = b ? (const uint32_t *)b + i * nl : nullptr;
+        const uint32_t *pc = c ? (const uint32_t *)c + i * nl : nullptr;
+        uint32_t *pd = (uint32_t *)d + i * nl;
+        uint32_t fl = 0;
+        switch (fmt) {
+        case 0: one<1>(op, rnd, pa, pb, pc, pd, &fl); break;
+        case 1: one<2>(op, rnd, pa, pb, pc, pd, &fl); break;
+        case 2: one<4>(op, rnd, pa, pb, pc, pd, &fl); break;
+        default: one<8>(op, rnd, pa, pb, pc, pd, &fl); break;
+        }
+        acc |= fl;
+    }
+    *flags_out = acc;
+    return 0;
+}
+
+extern "C" int emu_one(int fmt, int op, int rnd, const void *a, const void *b, const void *c,
+                       void *d, uint32_t *flags)
+{
+    return emu_run(fmt, op, rnd, a, b, c, d, 1, flags);
+}

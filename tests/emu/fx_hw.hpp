@@ -1,0 +1,30 @@
+// Portable stand-ins for the two gfx950 instructions of the device limb
+// arithmetic (cft-fp256_amd/csrc/fx_hw.hpp), so that the same fx_*.hpp
+// headers compile with g++ and the test suite can check their logic against
+// the oracle without a GPU.  Test infrastructure only.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#define __device__
+#define __forceinline__ inline
+#define __constant__
+#define __global__
+
+namespace fx {
+
+inline uint32_t alignbit(uint32_t hi, uint32_t lo, uint32_t s)
+{
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> (s & 31u));
+}
+
+inline void mac(uint64_t &acc, uint32_t &hi, uint32_t a, uint32_t b)
+{
+    const uint64_t p = (uint64_t)a * b;
+    const uint64_t s = acc + p;
+    hi += s < p ? 1u : 0u;
+    acc = s;
+}
+
+} // namespace fx
