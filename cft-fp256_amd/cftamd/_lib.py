@@ -173,6 +173,9 @@ def _bind(lib):
         "cft_save_all_flags": ([V], c_uint32),
         "cft_restore_flags": ([V, c_uint32, c_uint32], None),
         "cft_test_saved_flags": ([c_uint32, c_uint32], c_int),
+        "cftx_kernel_timer": ([V, c_int], c_int),
+        "cftx_kernel_timer_read": ([V, POINTER(ctypes.c_double),
+                                    POINTER(ctypes.c_uint64)], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -267,6 +267,66 @@ cft_status bind_out(cft_device *dev, int role, void *p, size_t bytes, void **dpt
     return CFT_OK;
 }
 
+// ---- kernel timer (include/cft_mi355x.h) -------------------------------------
+
+hipError_t timer_begin(cft_device *dev)
+{
+    return dev->timing ? hipEventRecord(dev->ev0, dev->stream) : hipSuccess;
+}
+
+hipError_t timer_end(cft_device *dev)
+{
+    return dev->timing ? hipEventRecord(dev->ev1, dev->stream) : hipSuccess;
+}
+
+void timer_collect(cft_device *dev)
+{
+    if (!dev->timing)
+        return;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, dev->ev0, dev->ev1) == hipSuccess) {
+        dev->timed_ms += ms;
+        dev->timed_launches++;
+    }
+    (void)hipGetLastError();
+}
+
+extern "C" CFT_API cft_status cftx_kernel_timer(cft_device *dev, int enable)
+{
+    cft_clear_error();
+    if (!dev) {
+        cft_set_error("cftx_kernel_timer: dev is NULL");
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    cft_status st = dev_enter(dev, "cftx_kernel_timer");
+    if (st != CFT_OK)
+        return st;
+    if (enable && !dev->ev0) {
+        if (hipEventCreate(&dev->ev0) != hipSuccess || hipEventCreate(&dev->ev1) != hipSuccess) {
+            (void)hipGetLastError();
+            cft_set_error("cftx_kernel_timer: creating HIP events failed");
+            return CFT_ERR_OUT_OF_MEMORY;
+        }
+    }
+    dev->timing = enable != 0;
+    dev->timed_ms = 0.0;
+    dev->timed_launches = 0;
+    return CFT_OK;
+}
+
+extern "C" CFT_API cft_status cftx_kernel_timer_read(cft_device *dev, double *total_ms,
+                                                     uint64_t *launches)
+{
+    cft_clear_error();
+    if (!dev || !total_ms || !launches) {
+        cft_set_error("cftx_kernel_timer_read: a NULL argument");
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    *total_ms = dev->timed_ms;
+    *launches = dev->timed_launches;
+    return CFT_OK;
+}
+
 // ---- open / close -----------------------------------------------------------
 
 static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft_device **out)
@@ -409,6 +469,8 @@ extern "C" CFT_API void cft_close(cft_device *dev)
         if (a.p) (void)hipFree(a.p);
     if (dev->dwords) (void)hipFree(dev->dwords);
     if (dev->hwords) (void)hipHostFree(dev->hwords);
+    if (dev->ev0) (void)hipEventDestroy(dev->ev0);
+    if (dev->ev1) (void)hipEventDestroy(dev->ev1);
     (void)hipStreamDestroy(dev->stream);
     (void)hipGetLastError();
     delete dev;

@@ -61,6 +61,8 @@ cft_status execute(const char *who, cft_device *dev, int which, int op, int fmt,
         (st = bind_out(dev, ROLE_D, d, n * esz, &dd, &dbuf, who)) != CFT_OK)
         return st;
     hipError_t e = cftk_fill32(dev->dwords, 0u, 1, dev->stream);
+    if (e == hipSuccess)
+        e = timer_begin(dev);
     if (e == hipSuccess) {
         const int grid = grid_for(dev, n);
         e = which < 0 ? cftk_elem(fmt, op, rnd, da, db, dc, dd, n, smask, dev->dwords, grid,
@@ -68,6 +70,8 @@ cft_status execute(const char *who, cft_device *dev, int which, int op, int fmt,
                       : cftk_divsqrt(fmt, which, rnd, da, db, dd, n, smask, dev->dwords, grid,
                                      dev->stream);
     }
+    if (e == hipSuccess)
+        e = timer_end(dev);
     if (e != hipSuccess)
         return dev_fail(dev, e, who, "launching the kernel");
     if (!dbuf) {
@@ -80,6 +84,7 @@ cft_status execute(const char *who, cft_device *dev, int which, int op, int fmt,
         e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess)
         return dev_fail(dev, e, who, "running the kernel");
+    timer_collect(dev);
     if (dbuf)
         dbuf->host_valid = false;               // the device copy is the truth now
     cft_emit_flags(dev, dev->hwords[0], flags_out);

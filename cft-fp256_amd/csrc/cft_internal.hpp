@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/cft.h"
+#include "../../include/cft_mi355x.h"
 
 // The library's own limits, published through cft_get_caps.
 namespace cftl {
@@ -54,7 +55,19 @@ struct cft_device {
     uint32_t *dwords = nullptr;    // device: [0] flags, [1] status
     uint32_t *hwords = nullptr;    // pinned mirror of dwords
     std::vector<cft_buffer *> bufs;
+    // cftx_kernel_timer: events around each compute kernel on `stream`
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double timed_ms = 0.0;
+    uint64_t timed_launches = 0;
 };
+
+// Bracket a compute-kernel launch with the handle's timer events (no-ops
+// unless cftx_kernel_timer enabled them); timer_end adds the elapsed time
+// once the stream has been synchronised.
+hipError_t timer_begin(cft_device *dev);
+hipError_t timer_end(cft_device *dev);
+void timer_collect(cft_device *dev);
 
 // ---- the error slot (process-global, as the reference's) ----------------
 void cft_clear_error();

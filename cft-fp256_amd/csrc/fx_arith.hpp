@@ -44,9 +44,13 @@ template <int NL> struct F {
     static constexpr int TOPF = MANW - 32 * (NL - 1); // fraction bits, top limb
     static constexpr uint32_t EMASK = (1u << EXPW) - 1u;
     static constexpr uint32_t FMASK = (1u << TOPF) - 1u;
-    // FMA window: jam bit, 2P product bits, P + 2 bits of addend headroom
-    // above the product, a carry bit.
-    static constexpr int WL = (3 * P + 4 + 31) / 32;
+    // FMA window (2NL limbs): the bit T where the larger term's leading bit
+    // is placed.  Chosen per format so that the product's placement is a limb
+    // move plus a 1..32-bit shift, the three possible rounding positions
+    // T-P .. T-P+2 share a limb with the guard bit, bit 0 stays below every
+    // real bit (the jam), and T+2 stays under the sign bit 64NL-1.
+    static constexpr int T = NL == 1 ? 57 : NL == 2 ? 120 : NL == 4 ? 248 : 504;
+    static_assert(T + 2 <= 64 * NL - 1, "no room above the window's top term");
 };
 
 // ---- limb primitives ------------------------------------------------------
@@ -100,12 +104,15 @@ __device__ __forceinline__ bool is_zero(const uint32_t (&x)[N])
     return o == 0;
 }
 
-// x <<= 32*k limbs, zero fill; 0 <= k < 32.
+// x <<= 32*k limbs, zero fill; 0 <= k <= N (a select level per power of
+// two up to N: a shift by N limbs empties the array).
 template <int N>
 __device__ __forceinline__ void limb_shl(uint32_t (&x)[N], uint32_t k)
 {
 #pragma unroll
     for (int t = 16; t >= 1; t >>= 1) {
+        if (t > N)
+            continue;
         const bool s = (k & (uint32_t)t) != 0;
 #pragma unroll
         for (int i = N - 1; i >= 0; i--)
@@ -113,12 +120,14 @@ __device__ __forceinline__ void limb_shl(uint32_t (&x)[N], uint32_t k)
     }
 }
 
-// x >>= 32*k limbs, the dropped limbs ORed into jam; 0 <= k < 32.
+// x >>= 32*k limbs, the dropped limbs ORed into jam; 0 <= k <= N.
 template <int N>
 __device__ __forceinline__ void limb_shr_jam(uint32_t (&x)[N], uint32_t k, uint32_t &jam)
 {
 #pragma unroll
     for (int t = 16; t >= 1; t >>= 1) {
+        if (t > N)
+            continue;
         const bool s = (k & (uint32_t)t) != 0;
         uint32_t dropped = 0;
 #pragma unroll
@@ -169,13 +178,13 @@ template <int NL> struct Unp {
     uint32_t m[NL];     // significand, normalised: bit P-1 set when nonzero
     int e;              // exponent of m's lsb
     int sign;
-    bool zero, inf, nan, snan;
+    bool zero, inf, nan, snan, sub;
 };
 
-// host/src/softfloat.c:85-116, then a subnormal significand normalised so
-// its leading bit is at P-1 (its exponent lowered to match).
+// host/src/softfloat.c:85-116.  A subnormal keeps its raw fraction here and
+// sets `sub`; normalise() below moves its leading bit to P-1.
 template <int NL>
-__device__ __forceinline__ void unpack(const uint32_t (&x)[NL], Unp<NL> &u)
+__device__ __forceinline__ void unpack_raw(const uint32_t (&x)[NL], Unp<NL> &u)
 {
     using Fm = F<NL>;
     const uint32_t top = x[NL - 1];
@@ -189,17 +198,33 @@ __device__ __forceinline__ void unpack(const uint32_t (&x)[NL], Unp<NL> &u)
     u.snan = u.nan && !((top >> (Fm::TOPF - 1)) & 1u);
     u.inf = ef == Fm::EMASK && fnz == 0;
     u.zero = ef == 0 && fnz == 0;
+    u.sub = ef == 0 && fnz != 0;
 #pragma unroll
     for (int i = 0; i < NL - 1; i++)
         u.m[i] = x[i];
     u.m[NL - 1] = (top & Fm::FMASK) | (ef ? (1u << Fm::TOPF) : 0u);
     u.e = (ef ? (int)ef : 1) - Fm::BIAS - Fm::MANW;
-    if (ef == 0 && fnz != 0) {
-        // subnormal: rare, so a divergent branch is the cheap form
+}
+
+// A subnormal's significand shifted so its leading bit is at P-1, its
+// exponent lowered to match; any other value is left alone.
+template <int NL>
+__device__ __forceinline__ void normalise(Unp<NL> &u)
+{
+    using Fm = F<NL>;
+    if (u.sub) {
         const int L = bitlen<NL>(u.m);
         shl<NL>(u.m, (uint32_t)(Fm::P - L));
         u.e -= Fm::P - L;
+        u.sub = false;
     }
+}
+
+template <int NL>
+__device__ __forceinline__ void unpack(const uint32_t (&x)[NL], Unp<NL> &u)
+{
+    unpack_raw<NL>(x, u);
+    normalise<NL>(u);
 }
 
 template <int NL>
@@ -320,13 +345,12 @@ __device__ __forceinline__ void round_pack_norm(int sign, uint32_t (&K)[NL], boo
     const bool carry_u = allones && round_incr(rnd, sign, g, s, K[0] & 1u);
     const bool tiny = !(e_norm == Fm::EMIN - 1 && carry_u);
     // Denormalise X = K.g by sh = emin - e_norm places, sticky folded.
-    uint32_t X[NL + 1];
-    X[0] = g ? 1u : 0u;                         // X = (K << 1) | g ... in two words
+    uint32_t X[NL + 1];                          // X = (K << 1) | g
+    X[0] = (K[0] << 1) | (g ? 1u : 0u);
 #pragma unroll
-    for (int i = 0; i < NL; i++) {
-        X[i] |= K[i] << 1;
-        X[i + 1] = K[i] >> 31;
-    }
+    for (int i = 1; i < NL; i++)
+        X[i] = alignbit(K[i], K[i - 1], 31);
+    X[NL] = K[NL - 1] >> 31;
     uint32_t jam = s ? 1u : 0u;
     shr_jam<NL + 1>(X, (uint32_t)(Fm::EMIN - e_norm), jam);
     const bool g2 = X[0] & 1u, s2 = jam != 0;
@@ -376,51 +400,110 @@ __device__ __forceinline__ void round_window(int sign, uint32_t (&S)[N], int eW,
 
 // ---- fused multiply-add -----------------------------------------------------
 
-// a*b + c for the steered operands of OP, one rounding.  host/src/softfloat.c:
-// 324-428 (special-case order: NaN, inf*0, inf-inf, inf, c inf, a zero factor)
-// and 438-468 (ADD = a*1 + c, SUB = a*1 - c, MUL = a*b + (+-0)).
-//
-// Method.  Subnormal operands are normalised first (a rare branch), so the
-// exact product P = ma*mb has its leading bit at 2P-2 or 2P-1.  The exact sum
-// is formed in a window of WL limbs: bit 0 is reserved for a jam (sticky)
-// bit, P sits at bits [1, 2P+1) unshifted, and the addend C is placed by ONE
-// variable right shift of C << (2P+3), every bit that falls to bit 0 or below
-// jammed into bit 0.  When C leads P by more than P+2 binades P is below
-// every rounding boundary of the result and is itself replaced by the jam bit
-// (C then sits at its top placement).  Jamming is exact in effect: the
-// representative lies in the same open interval between consecutive rounding
-// boundaries, which are all multiples of 2 in window units.
-template <int NL, int OP>
-__device__ __forceinline__ void fma_op(const uint32_t (&xa)[NL], const uint32_t (&xb)[NL],
-                                       const uint32_t (&xc)[NL], int rnd, uint32_t (&out)[NL],
-                                       uint32_t &flags)
+// Carry chains: one v_add_co/v_addc (v_sub_co/v_subb) per limb.
+template <int N>
+__device__ __forceinline__ uint32_t add_limbs(uint32_t (&r)[N], const uint32_t (&x)[N],
+                                              const uint32_t (&y)[N], uint32_t cin)
+{
+    uint32_t c = cin;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        uint32_t co;
+        r[i] = addc(x[i], y[i], c, &co);
+        c = co;
+    }
+    return c;
+}
+
+// x = -x (two's complement), N limbs.
+template <int N>
+__device__ __forceinline__ void neg_limbs(uint32_t (&x)[N])
+{
+    uint32_t c = 1;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        uint32_t co;
+        x[i] = addc(~x[i], 0u, c, &co);
+        c = co;
+    }
+}
+
+// Round the exact value held in S (N limbs) whose leading bit is at bit M,
+// M in {T-1, T, T+1} for a T fixed at compile time, e_norm the exponent of
+// that leading bit.  The three possible rounding positions M-P+1 share one
+// limb, so K is one variable v_alignbit per limb.
+template <int NL, int N, int T>
+__device__ __forceinline__ void round_fixed(int sign, const uint32_t (&S)[N], int M, int e_norm,
+                                            int rnd, uint32_t (&out)[NL], uint32_t &flags)
 {
     using Fm = F<NL>;
-    constexpr int P = Fm::P, WL = Fm::WL;
+    constexpr int P = Fm::P;
+    constexpr int LO = T - P;                    // lowest K position (M = T-1)
+    constexpr int LI = LO / 32;
+    static_assert((T - P + 2) / 32 == LI, "rounding positions straddle a limb");
+    static_assert(LO % 32 >= 1, "guard bit must share the limb");
+    static_assert(LI + NL < N + 1, "window too narrow");
+    const uint32_t off = (uint32_t)(M - P + 1 - 32 * LI);   // 1..31
+    uint32_t K[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        K[i] = alignbit(LI + i + 1 < N ? S[LI + i + 1] : 0u, S[LI + i], off);
+    K[NL - 1] &= (2u << Fm::TOPF) - 1u;          // the P bits, nothing above
+    const bool g = (S[LI] >> (off - 1)) & 1u;
+    uint32_t sticky = S[LI] & ((1u << (off - 1)) - 1u);
+#pragma unroll
+    for (int i = 0; i < LI; i++)
+        sticky |= S[i];
+    round_pack_norm<NL>(sign, K, g, sticky != 0, e_norm, rnd, out, flags);
+}
+
+// a*b + c for the steered operands of op, one rounding.  host/src/softfloat.c:
+// 324-428 (special-case order: NaN, inf*0, inf-inf, inf, c inf, a zero factor)
+// and 438-468 (ADD = a*1 + c, SUB = a*1 - c, MUL = a*b + (+-0)).  op is
+// uniform across the wave (a kernel constant, or an interpreted instruction).
+//
+// Method.  Subnormal operands are normalised first (one rare branch), so the
+// exact product P = ma*mb has its leading bit at 2P-2 or 2P-1.  The exact sum
+// is formed in a window of 2NL limbs in which the LARGER term - by leading-bit
+// exponent - sits with its leading bit at a fixed bit T = 64NL-7, and the
+// smaller term is placed by one variable right shift, every bit shifted past
+// the bottom jammed into bit 0 (which no real bit of either term occupies).
+// Jamming is exact in effect: the representative lies in the same open
+// interval between consecutive rounding boundaries.  The sum's leading bit is
+// then at T-1, T or T+1 unless the terms were within a binade of each other
+// and of opposite sign (deep cancellation), which takes a rare branch that
+// normalises by a full shift.
+template <int NL>
+__device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const uint32_t (&xb)[NL],
+                                         const uint32_t (&xc)[NL], int rnd, uint32_t (&out)[NL],
+                                         uint32_t &flags)
+{
+    using Fm = F<NL>;
+    constexpr int P = Fm::P, WN = 2 * NL, T = Fm::T;
     Unp<NL> ua, ub, uc;
-    unpack<NL>(xa, ua);
-    if (OP == OP_ADD || OP == OP_SUB) {          // b := 1.0
+    unpack_raw<NL>(xa, ua);
+    if (op == OP_ADD || op == OP_SUB) {          // b := 1.0
 #pragma unroll
         for (int i = 0; i < NL - 1; i++)
             ub.m[i] = 0;
         ub.m[NL - 1] = 1u << Fm::TOPF;
         ub.e = -Fm::MANW;
         ub.sign = 0;
-        ub.zero = ub.inf = ub.nan = ub.snan = false;
+        ub.zero = ub.inf = ub.nan = ub.snan = ub.sub = false;
     } else {
-        unpack<NL>(xb, ub);
+        unpack_raw<NL>(xb, ub);
     }
-    if (OP == OP_MUL) {                          // c := +-0, sign of the product
+    if (op == OP_MUL) {                          // c := +-0, sign of the product
 #pragma unroll
         for (int i = 0; i < NL; i++)
             uc.m[i] = 0;
         uc.e = 0;
         uc.sign = ua.sign ^ ub.sign;
         uc.zero = true;
-        uc.inf = uc.nan = uc.snan = false;
+        uc.inf = uc.nan = uc.snan = uc.sub = false;
     } else {
-        unpack<NL>(xc, uc);
-        if (OP == OP_SUB)
+        unpack_raw<NL>(xc, uc);
+        if (op == OP_SUB)
             uc.sign ^= 1;
     }
     const int sp = ua.sign ^ ub.sign;
@@ -445,118 +528,134 @@ __device__ __forceinline__ void fma_op(const uint32_t (&xa)[NL], const uint32_t 
 #pragma unroll
             for (int i = 0; i < NL - 1; i++)
                 out[i] = xc[i];
-            out[NL - 1] = xc[NL - 1] ^ (OP == OP_SUB ? 0x80000000u : 0u);
+            out[NL - 1] = xc[NL - 1] ^ (op == OP_SUB ? 0x80000000u : 0u);
         }
         return;
     }
+    if (ua.sub || ub.sub || uc.sub) {            // rare: normalise subnormals
+        normalise<NL>(ua);
+        normalise<NL>(ub);
+        normalise<NL>(uc);
+    }
 
-    uint32_t Pm[2 * NL];
-    if (OP == OP_ADD || OP == OP_SUB) {
-        // ma * 2^(P-1): a constant shift instead of a product
+    // the exact product, leading bit at 2P-2 or 2P-1
+    uint32_t Pm[WN];
+    if (op == OP_ADD || op == OP_SUB) {
 #pragma unroll
-        for (int i = 0; i < 2 * NL; i++)
+        for (int i = 0; i < WN; i++)
             Pm[i] = 0;
         constexpr int LS = (P - 1) / 32, BS = (P - 1) % 32;
 #pragma unroll
         for (int i = 0; i <= NL; i++) {
             const uint32_t lo = (i - 1 >= 0) ? ua.m[i - 1] : 0u;
             const uint32_t hi = (i < NL) ? ua.m[i] : 0u;
-            if (i + LS < 2 * NL)
-                Pm[i + LS] = BS ? ((hi << BS) | (lo >> (32 - BS))) : hi;
+            if (i + LS < WN)
+                Pm[i + LS] = BS ? alignbit(hi, lo, 32u - BS) : hi;
         }
     } else {
         mul_limbs<NL, NL>(ua.m, ub.m, Pm);
     }
     const int ep = ua.e + ub.e;
+    constexpr int TB = 2 * P - 1;                // the product's possible top bit
+    const bool ptop = (Pm[TB / 32] >> (TB % 32)) & 1u;
 
-    if (OP == OP_MUL) {
-        round_window<NL, 2 * NL>(sp, Pm, ep, rnd, out, flags);
+    if (op == OP_MUL) {
+        // leading bit at 2P-1 or 2P-2: round from that fixed position
+        round_fixed<NL, WN, 2 * P - 2>(sp, Pm, ptop ? 2 * P - 1 : 2 * P - 2,
+                                       ep + (ptop ? 2 * P - 1 : 2 * P - 2), rnd, out, flags);
         return;
     }
 
-    // window
-    constexpr int TB = 2 * P - 1;                // the product's possible top bit
-    const int Lp = ((Pm[TB / 32] >> (TB % 32)) & 1u) ? 2 * P : 2 * P - 1;
-    const int VEp = ep + Lp - 1;
-    const bool czero = uc.zero;
-    const int VEc = uc.e + P - 1;
-    const bool cdom = !czero && (VEc - VEp > P + 2);
-    uint32_t S[WL], Cw[WL];
-#pragma unroll
-    for (int i = 0; i < WL; i++)
-        S[i] = 0;
-    // P << 1, or the jam bit when C dominates
-#pragma unroll
-    for (int i = 0; i < 2 * NL; i++)
-        S[i] = (Pm[i] << 1) | (i > 0 ? Pm[i - 1] >> 31 : 0u);
-    if (2 * NL < WL)
-        S[2 * NL] = Pm[2 * NL - 1] >> 31;
-    if (cdom) {
-#pragma unroll
-        for (int i = 0; i < WL; i++)
-            S[i] = 0;
-        S[0] = 1;
-    }
-    // C << (2P+3), then right by r
+    // Pw = P with its leading bit at T: a shift of T-TB or T-TB+1 bits
+    uint32_t Pw[WN], Cw[WN];
     {
-        constexpr int CP = 2 * P + 3, CL = CP / 32, CB = CP % 32;
+        constexpr int SH = T - TB;               // when ptop
+        static_assert(SH >= 1, "no room for the jam bit");
+        const uint32_t s = ptop ? (uint32_t)SH : (uint32_t)(SH + 1);
+        // constant limb part and a 0/1-bit remainder: (SH, SH+1) = 32q + (r, r+1)
+        constexpr int Q = SH / 32, R = SH % 32;
+        static_assert(R >= 1 && R + 1 <= 32, "v_alignbit takes a shift of 0..31");
+        const uint32_t r = s - 32u * Q;          // R or R+1, >= 1
 #pragma unroll
-        for (int i = 0; i < WL; i++)
-            Cw[i] = 0;
-#pragma unroll
-        for (int i = 0; i <= NL; i++) {
-            const uint32_t lo = (i - 1 >= 0) ? uc.m[i - 1] : 0u;
-            const uint32_t hi = (i < NL) ? uc.m[i] : 0u;
-            if (i + CL < WL)
-                Cw[i + CL] = CB ? ((hi << CB) | (lo >> (32 - CB))) : hi;
+        for (int i = 0; i < WN; i++) {
+            const int j = i - Q;
+            const uint32_t hi = (j >= 0 && j < WN) ? Pm[j] : 0u;
+            const uint32_t lo = (j - 1 >= 0 && j - 1 < WN) ? Pm[j - 1] : 0u;
+            Pw[i] = alignbit(hi, lo, 32u - r);
         }
-        const int rr = cdom ? 0 : (2 * P + 2 - (uc.e - ep));
-        uint32_t jam = 0;
-        shr_jam<WL>(Cw, czero ? 0u : (uint32_t)rr, jam);
-        Cw[0] |= jam ? 1u : 0u;
     }
-    const int eW = cdom ? (uc.e - (2 * P + 3)) : (ep - 1);
-    int sign;
-    if (uc.sign == sp) {
-        uint64_t c = 0;
+    const int VEp = ep + (ptop ? 2 * P - 1 : 2 * P - 2);
+    // Cw = C with its leading bit at T (a constant shift)
+    {
+        constexpr int SH = T - (P - 1), Q = SH / 32, R = SH % 32;
 #pragma unroll
-        for (int i = 0; i < WL; i++) {
-            c += (uint64_t)S[i] + Cw[i];
-            S[i] = (uint32_t)c;
-            c >>= 32;
+        for (int i = 0; i < WN; i++) {
+            const int j = i - Q;
+            const uint32_t hi = (j >= 0 && j < NL) ? uc.m[j] : 0u;
+            const uint32_t lo = (j - 1 >= 0 && j - 1 < NL) ? uc.m[j - 1] : 0u;
+            Cw[i] = R ? alignbit(hi, lo, 32u - R) : hi;
         }
-        sign = sp;
-    } else {
-        // D = C - P; negative: take P - C
-        uint32_t D[WL];
-        uint32_t br = 0;
+    }
+    // a zero addend is below everything: the product alone, exactly
+    const int VEc = uc.zero ? VEp - 4 * WN * 32 : uc.e + P - 1;
+    const int d = VEc - VEp;
+    const bool cbig = d > 0;
+    uint32_t big[WN], small[WN];
 #pragma unroll
-        for (int i = 0; i < WL; i++) {
-            const uint64_t t = (uint64_t)Cw[i] - S[i] - br;
-            D[i] = (uint32_t)t;
-            br = (uint32_t)(t >> 63);
+    for (int i = 0; i < WN; i++) {
+        big[i] = cbig ? Cw[i] : Pw[i];
+        small[i] = cbig ? Pw[i] : Cw[i];
+    }
+    const int bsign = cbig ? uc.sign : sp;
+    const bool subtract = uc.sign != sp;
+    {
+        uint32_t jam = 0;
+        const int ad = d < 0 ? -d : d;
+        shr_jam<WN>(small, (uint32_t)(ad > 32 * WN ? 32 * WN : ad), jam);
+        small[0] |= jam ? 1u : 0u;
+    }
+    const int VEb = cbig ? VEc : VEp;            // the exponent of bit T
+    uint32_t S[WN];
+    {
+        const uint32_t m = subtract ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+        for (int i = 0; i < WN; i++)
+            small[i] ^= m;
+        add_limbs<WN>(S, big, small, subtract ? 1u : 0u);
+    }
+    // leading bit: T+1 (carry), T, T-1 (borrow) - or lower / negative
+    const bool t1 = (S[(T + 1) / 32] >> ((T + 1) % 32)) & 1u;
+    const bool t0 = (S[T / 32] >> (T % 32)) & 1u;
+    const bool tm = (S[(T - 1) / 32] >> ((T - 1) % 32)) & 1u;
+    const bool neg = (S[WN - 1] >> 31) & 1u;     // only when d == 0, subtracting
+    int sign = bsign;
+    int M;
+    if (neg || (!t1 && !t0 && !tm)) {
+        // rare: equal leading exponents cancelled (or the smaller term won)
+        if (neg) {
+            neg_limbs<WN>(S);
+            sign ^= 1;
         }
-        if (br) {
-            uint32_t b2 = 0;
-#pragma unroll
-            for (int i = 0; i < WL; i++) {
-                const uint64_t t = (uint64_t)0 - D[i] - b2;
-                S[i] = (uint32_t)t;
-                b2 = (uint32_t)(t >> 63);
-            }
-            sign = sp;
-        } else {
-#pragma unroll
-            for (int i = 0; i < WL; i++)
-                S[i] = D[i];
-            sign = uc.sign;
-        }
-        if (is_zero<WL>(S)) {                    // exact cancellation
+        const int L = bitlen<WN>(S);
+        if (L == 0) {                            // exact cancellation
             enc_zero<NL>(out, rnd == RDN);
             return;
         }
+        shl<WN>(S, (uint32_t)(T + 1 - (L - 1)));  // leading bit to T+1
+        M = T + 1;
+        round_fixed<NL, WN, T>(sign, S, M, VEb + (L - 1) - T, rnd, out, flags);
+        return;
     }
-    round_window<NL, WL>(sign, S, eW, rnd, out, flags);
+    M = t1 ? T + 1 : (t0 ? T : T - 1);
+    round_fixed<NL, WN, T>(sign, S, M, VEb + (M - T), rnd, out, flags);
+}
+
+template <int NL, int OP>
+__device__ __forceinline__ void fma_op(const uint32_t (&xa)[NL], const uint32_t (&xb)[NL],
+                                       const uint32_t (&xc)[NL], int rnd, uint32_t (&out)[NL],
+                                       uint32_t &flags)
+{
+    fma_core<NL>(OP, xa, xb, xc, rnd, out, flags);
 }
 
 } // namespace fx

@@ -194,7 +194,7 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
 #pragma unroll
         for (int i = 0; i < k; i++) {
             const uint32_t lo = i > 0 ? r[NL - k + i - 1] : 0u;
-            V[i] = (r[NL - k + i] << 1) | (lo >> 31);
+            V[i] = alignbit(r[NL - k + i], lo, 31);
         }
         V[k] = r[NL - 1] >> 31;
         // estimate d = R / (V * B), from the top limbs as doubles

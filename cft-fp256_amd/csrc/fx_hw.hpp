@@ -8,10 +8,16 @@
 
 namespace fx {
 
-// ((hi:lo) >> (s & 31)) [31:0]: v_alignbit_b32
+// ((hi:lo) >> (s & 31)) [31:0]: v_alignbit_b32.  Written as the instruction
+// itself: hipcc 7.2's gfx950 backend has been seen to lower a funnel shift of
+// limb x by k to v_alignbit on the register holding x << k when x << k is
+// also live (the IR was right, the ISA was not), so every multi-limb shift in
+// this library goes through this opaque form.
 __device__ __forceinline__ uint32_t alignbit(uint32_t hi, uint32_t lo, uint32_t s)
 {
-    return __builtin_amdgcn_alignbit(hi, lo, s);
+    uint32_t r;
+    asm("v_alignbit_b32 %0, %1, %2, %3" : "=v"(r) : "v"(hi), "v"(lo), "v"(s));
+    return r;
 }
 
 // acc64 += a * b, the carry out of bit 63 added into hi: one v_mad_u64_u32
@@ -22,6 +28,12 @@ __device__ __forceinline__ void mac(uint64_t &acc, uint32_t &hi, uint32_t a, uin
     uint64_t cc;
     asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc), "=s"(cc) : "v"(a), "v"(b));
     asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(hi), "=s"(cc) : "s"(cc));
+}
+
+// x + y + cin (cin 0 or 1), carry out: one v_add_co / v_addc in a chain.
+__device__ __forceinline__ uint32_t addc(uint32_t x, uint32_t y, uint32_t cin, uint32_t *cout)
+{
+    return __builtin_addc(x, y, cin, cout);
 }
 
 } // namespace fx

@@ -27,4 +27,11 @@ inline void mac(uint64_t &acc, uint32_t &hi, uint32_t a, uint32_t b)
     acc = s;
 }
 
+inline uint32_t addc(uint32_t x, uint32_t y, uint32_t cin, uint32_t *cout)
+{
+    const uint64_t s = (uint64_t)x + y + cin;
+    *cout = (uint32_t)(s >> 32);
+    return (uint32_t)s;
+}
+
 } // namespace fx
