@@ -8,7 +8,7 @@ package binds it with ctypes, as the reference's own binding does.
 from . import _lib
 from ._lib import (CftError, FP32, FP64, FP128, FP256, RNE, RTZ, RDN, RUP, RMM,
                    check, lib, library_path)
-from .core import Buffer, Device
+from .core import Buffer, Device, Program, check_image
 
-__all__ = ["CftError", "Device", "Buffer", "FP32", "FP64", "FP128", "FP256",
+__all__ = ["CftError", "Device", "Buffer", "Program", "check_image", "FP32", "FP64", "FP128", "FP256",
            "RNE", "RTZ", "RDN", "RUP", "RMM", "check", "lib", "library_path", "_lib"]

@@ -167,6 +167,8 @@ def _bind(lib):
         "cft_program_run_bank": ([V, V, c_size_t, V, V, V, V, P, c_size_t,
                                   P, P], c_int),
         "cft_program_run_ex": ([V, POINTER(CftRunArgs)], c_int),
+        "cft_program_digest": ([V, V, c_size_t, V], c_int),
+        "cftx_program_check": ([V, c_size_t, c_uint32, POINTER(CftProgramInfo)], c_int),
         "cft_lower_flags": ([V, c_uint32], None),
         "cft_raise_flags": ([V, c_uint32], None),
         "cft_test_flags": ([V, c_uint32], c_int),

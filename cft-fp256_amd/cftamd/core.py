@@ -111,6 +111,93 @@ class Device:
     def alloc(self, nbytes):
         return Buffer(self, nbytes)
 
+    def load(self, image):
+        """A program from its image bytes (cft_program_load)."""
+        return Program(self, image)
+
+
+def _u32p(x):
+    if x is None:
+        return None
+    p = _ptr(x)
+    return ctypes.cast(p, ctypes.POINTER(c_uint32))
+
+
+class Program:
+    """A loaded sequencer program (cft_program_load) and its runs
+    (cft_program_run_ex), with the run's arrays as numpy uint8 / uint32
+    arrays or resident buffers, laid out as cft_run_args has them."""
+
+    def __init__(self, dev, image):
+        self.dev = dev
+        self._image = np.frombuffer(bytes(image), dtype=np.uint8).copy()
+        h = c_void_p()
+        check(lib().cft_program_load(dev.handle, self._image.ctypes.data, self._image.nbytes,
+                                     byref(h)), "cft_program_load")
+        self.handle = h
+
+    def info(self):
+        i = _lib.CftProgramInfo()
+        i.struct_size = ctypes.sizeof(i)
+        check(lib().cft_program_get_info(self.handle, byref(i)), "cft_program_get_info")
+        return i
+
+    def digest(self, bank=None):
+        out = (ctypes.c_uint8 * 32)()
+        nb = 0 if bank is None else (bank.nbytes if isinstance(bank, np.ndarray) else bank.bytes)
+        check(lib().cft_program_digest(self.handle, _ptr(bank), nb, out), "cft_program_digest")
+        return bytes(out)
+
+    def run(self, n, a, b=None, c=None, bank=None, scratch_in=None, scratch_out=None,
+            deposits=None, counts=None, lane_mask=None, lane_flags=None, idx_a=None,
+            idx_b=None, idx_c=None, idx_a_src=0, idx_b_src=0, idx_c_src=0,
+            idx_scratch_in=None, idx_scratch_src=0):
+        """One cft_program_run_ex over n lanes; returns (flags, status)."""
+        def nbytes(x):
+            if x is None:
+                return 0
+            return x.nbytes if isinstance(x, np.ndarray) else x.bytes
+        A = _lib.CftRunArgs()
+        A.struct_size = ctypes.sizeof(A)
+        A.a, A.b, A.c, A.n = _ptr(a), _ptr(b), _ptr(c), n
+        A.bank, A.bank_bytes = _ptr(bank), nbytes(bank)
+        A.scratch_in, A.scratch_in_bytes = _ptr(scratch_in), nbytes(scratch_in)
+        A.scratch_out, A.scratch_out_bytes = _ptr(scratch_out), nbytes(scratch_out)
+        A.deposits = _ptr(deposits)
+        A.counts = _u32p(counts)
+        fl, bus = c_uint32(), c_uint32()
+        A.flags_out = ctypes.pointer(fl)
+        A.bus_out = ctypes.pointer(bus)
+        A.idx_a, A.idx_b, A.idx_c = _u32p(idx_a), _u32p(idx_b), _u32p(idx_c)
+        A.idx_a_src, A.idx_b_src, A.idx_c_src = idx_a_src, idx_b_src, idx_c_src
+        A.idx_scratch_in, A.idx_scratch_src = _u32p(idx_scratch_in), idx_scratch_src
+        A.lane_mask, A.lane_mask_bytes = _ptr(lane_mask), nbytes(lane_mask)
+        A.lane_flags, A.lane_flags_bytes = _ptr(lane_flags), nbytes(lane_flags)
+        check(lib().cft_program_run_ex(self.handle, byref(A)), "cft_program_run_ex")
+        return fl.value, bus.value
+
+    def free(self):
+        if self.handle:
+            lib().cft_program_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def check_image(image, scratch_depth=0):
+    """The loader's verdict on an image without a device (cftx_program_check):
+    (status, cft_program_info or None)."""
+    buf = np.frombuffer(bytes(image), dtype=np.uint8).copy()
+    i = _lib.CftProgramInfo()
+    i.struct_size = ctypes.sizeof(i)
+    st = lib().cftx_program_check(buf.ctypes.data if buf.nbytes else None, buf.nbytes,
+                                  scratch_depth, byref(i))
+    return st, (i if st == _lib.OK else None)
+
 
 class Buffer:
     """A resident buffer: a pinned host mirror (``.host``, a numpy view) and a

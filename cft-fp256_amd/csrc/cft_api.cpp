@@ -202,6 +202,29 @@ cft_buffer *find_buffer(cft_device *dev, const void *p, size_t bytes)
     return nullptr;
 }
 
+cft_status host_in(cft_device *dev, const void *p, const char *who)
+{
+    if (!p)
+        return CFT_OK;
+    const uint8_t *q = (const uint8_t *)p;
+    for (cft_buffer *b : dev->bufs) {
+        const uint8_t *h = (const uint8_t *)b->host;
+        if (q < h || q >= h + b->bytes || b->host_valid)
+            continue;
+        cft_status st = dev_enter(dev, who);
+        if (st != CFT_OK)
+            return st;
+        hipError_t e = hipMemcpyAsync(b->host, b->devp, b->bytes, hipMemcpyDeviceToHost,
+                                      dev->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(dev->stream);
+        if (e != hipSuccess)
+            return dev_fail(dev, e, who, "bringing a buffer's mirror up to date");
+        b->host_valid = true;
+    }
+    return CFT_OK;
+}
+
 // A buffer's device copy made current before a run binds it.
 static cft_status buffer_fresh(cft_buffer *b, const char *who, bool *staged)
 {
@@ -393,7 +416,10 @@ static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft
         return CFT_ERR_NO_DEVICE;
     }
     D->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    // default priority, always (see the stream-priority note in DESIGN.md)
+    if (prop.sharedMemPerBlock > 0)
+        D->lds_max = prop.sharedMemPerBlock;
+    // default priority, always: work on streams of different priorities can
+    // stall each other indefinitely on this platform
     if ((e = hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking)) != hipSuccess) {
         cft_set_error("cft_open: %s: creating a stream failed: %s", artifact,
                       hipGetErrorString(e));

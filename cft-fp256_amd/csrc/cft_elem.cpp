@@ -18,8 +18,6 @@
 
 namespace {
 
-enum { ROLE_A = 0, ROLE_B = 1, ROLE_C = 2, ROLE_D = 3 };
-
 bool is_reduction(int op) { return op == 24 || op == 25 || op == 28 || op == 29 || op == 31; }
 
 bool op_assigned(int op)

@@ -24,8 +24,18 @@ constexpr uint32_t kSeqFeatures =
     CFT_SEQ_FEAT_WIDE_CONST | CFT_SEQ_FEAT_REGS32 | CFT_SEQ_FEAT_BANK_PTR | CFT_SEQ_FEAT_KX9 |
     CFT_ALU_EXT_IMUL | CFT_SEQ_FEAT_SCRATCH | CFT_SEQ_FEAT_SCRATCH_IO |
     CFT_SEQ_FEAT_SCRATCH_STRICT | CFT_SEQ_FEAT_SCALAR | CFT_SEQ_FEAT_LANE_MASK |
+    CFT_SEQ_FEAT_INDEXED |
     CFT_SEQ_FEAT_SCRATCH_STEP | CFT_SEQ_FEAT_LANE_FLAGS | CFT_SEQ_FEAT_FLAG_CONTROL;
 }
+
+// Staging roles: each operand a call binds has its own growable arena.
+enum Role {
+    ROLE_A, ROLE_B, ROLE_C, ROLE_D,                 // streams and the result/deposits
+    ROLE_IA, ROLE_IB, ROLE_IC, ROLE_ISI,            // index tables
+    ROLE_BANK, ROLE_SIN, ROLE_SOUT, ROLE_COUNTS, ROLE_MASK, ROLE_LFLAGS,
+    ROLE_POOL, ROLE_TRIPS,
+    ROLE_COUNT
+};
 
 // One growable device allocation.
 struct DevArena {
@@ -51,7 +61,8 @@ struct cft_device {
     bool poisoned = false;
     uint32_t scratch_depth = cftl::kDefaultDepth;
     int num_cus = 256;
-    DevArena stage[8];             // per operand role
+    DevArena stage[ROLE_COUNT];    // per operand role
+    size_t lds_max = 65536;        // LDS bytes a workgroup may take
     uint32_t *dwords = nullptr;    // device: [0] flags, [1] status
     uint32_t *hwords = nullptr;    // pinned mirror of dwords
     std::vector<cft_buffer *> bufs;
@@ -101,6 +112,10 @@ cft_status bind_out(cft_device *dev, int role, void *p, size_t bytes, void **dpt
 
 // The buffer containing [p, p+bytes), or NULL.
 cft_buffer *find_buffer(cft_device *dev, const void *p, size_t bytes);
+
+// Before the host reads [p, p+bytes): a resident buffer whose start p points
+// into has its mirror brought up to date from the device if a run wrote it.
+cft_status host_in(cft_device *dev, const void *p, const char *who);
 
 // Kernel grid for n lanes of one element each.
 int grid_for(const cft_device *dev, size_t n);

@@ -153,7 +153,10 @@ __device__ __forceinline__ uint32_t top9(const uint32_t (&x)[NL])
     // fraction bits [MANW-9, MANW)
     constexpr int lo = Fm::MANW - 9;
     constexpr int li = lo / 32, bi = lo % 32;
-    const uint32_t w0 = x[li], w1 = (li + 1 < NL) ? x[li + 1] : 0u;
+    const uint32_t w0 = x[li];
+    uint32_t w1 = 0;
+    if constexpr (li + 1 < NL)
+        w1 = x[li + 1];
     return alignbit(w1, w0, bi) & 511u;
 }
 

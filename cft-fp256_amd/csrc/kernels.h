@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "seq_code.h"
+
 extern "C" {
 
 // d[i] = op(a[i], b[i], c[i]) over n elements of format fmt (0..3).  An
@@ -21,5 +23,16 @@ hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *
 
 // p[0..n) = v
 hipError_t cftk_fill32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
+
+// The sequencer interpreter over lanes [A->lane0, A->n) in workgroups of
+// `waves` wavefronts.  nregs: the program's registers after renaming;
+// phys_a/b/c: the renamed registers r0/r1/r2 (-1 where the program never
+// names them); trip_pc: the REPEAT whose count A->trips overrides per lane.
+hipError_t cftk_seq(int fmt, const SeqArgs *A, uint32_t nregs, int phys_a, int phys_b,
+                    int phys_c, uint32_t trip_pc, int waves, hipStream_t st);
+
+// LDS bytes one interpreter workgroup takes.
+size_t cftk_seq_lds(int fmt, uint32_t n_insns_in_lds, uint32_t n_consts, uint32_t nregs,
+                    int waves);
 
 }

@@ -247,6 +247,12 @@ typedef struct cft_run_args {
 CFT_API cft_status cft_program_run_ex(cft_program *prog,
                                       const cft_run_args *args);
 
+/* SHA-256 over the image bytes as loaded, then the bank's (NULL and 0 for a
+ * program that carries its own constants): reference host/include/cft.h:
+ * 3154-3160, host/src/program.c:2425-2454. */
+CFT_API cft_status cft_program_digest(cft_program *prog, const void *bank,
+                                      size_t bank_bytes, uint8_t out[32]);
+
 /* reference host/include/cft.h:3122-3135 */
 typedef struct cft_elem_args {
     size_t      struct_size;
