@@ -118,6 +118,19 @@ class CftElemArgs(ctypes.Structure):
                 ("idx_c_src", c_size_t)]
 
 
+class CftxBatchArgs(ctypes.Structure):
+    _fields_ = [("struct_size", c_size_t), ("bank", c_void_p), ("bank_bytes", c_size_t),
+                ("state", c_void_p), ("state_bytes", c_size_t),
+                ("steps", POINTER(c_uint32)), ("n", c_size_t),
+                ("a", c_void_p), ("b", c_void_p), ("c", c_void_p),
+                ("flags_out", POINTER(c_uint32)), ("bus_out", POINTER(c_uint32)),
+                ("lane_flags", c_void_p), ("lane_flags_bytes", c_size_t),
+                ("mode", c_int), ("lanes_per_wave", c_uint32),
+                ("waves_per_group", c_uint32), ("graph", c_int)]
+
+
+BATCH_PERSISTENT, BATCH_PER_OP = 0, 1
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 IN_TREE = os.path.join(os.path.dirname(HERE), "lib", "libcft.so")
 
@@ -169,6 +182,8 @@ def _bind(lib):
         "cft_program_run_ex": ([V, POINTER(CftRunArgs)], c_int),
         "cft_program_digest": ([V, V, c_size_t, V], c_int),
         "cftx_program_check": ([V, c_size_t, c_uint32, POINTER(CftProgramInfo)], c_int),
+        "cftx_program_run_batch": ([V, POINTER(CftxBatchArgs)], c_int),
+        "cftx_batch_launches": ([V], ctypes.c_uint64),
         "cft_lower_flags": ([V, c_uint32], None),
         "cft_raise_flags": ([V, c_uint32], None),
         "cft_test_flags": ([V, c_uint32], c_int),

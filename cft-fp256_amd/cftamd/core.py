@@ -176,6 +176,33 @@ class Program:
         check(lib().cft_program_run_ex(self.handle, byref(A)), "cft_program_run_ex")
         return fl.value, bus.value
 
+    def run_batch(self, state, steps, bank=None, a=None, b=None, c=None, lane_flags=None,
+                  mode=_lib.BATCH_PERSISTENT, lanes_per_wave=0, waves_per_group=0,
+                  graph=False):
+        """cftx_program_run_batch: lane i runs steps[i] trips of the segment
+        loop, its state (n x slots elements) updated in place; returns
+        (flags, status)."""
+        steps = np.ascontiguousarray(steps, dtype=np.uint32)
+        n = steps.size
+        nb = lambda x: 0 if x is None else (x.nbytes if isinstance(x, np.ndarray) else x.bytes)
+        A = _lib.CftxBatchArgs()
+        A.struct_size = ctypes.sizeof(A)
+        A.bank, A.bank_bytes = _ptr(bank), nb(bank)
+        A.state, A.state_bytes = _ptr(state), nb(state)
+        A.steps = steps.ctypes.data_as(ctypes.POINTER(c_uint32))
+        A.n = n
+        A.a, A.b, A.c = _ptr(a), _ptr(b), _ptr(c)
+        fl, bus = c_uint32(), c_uint32()
+        A.flags_out = ctypes.pointer(fl)
+        A.bus_out = ctypes.pointer(bus)
+        A.lane_flags, A.lane_flags_bytes = _ptr(lane_flags), nb(lane_flags)
+        A.mode = mode
+        A.lanes_per_wave = lanes_per_wave
+        A.waves_per_group = waves_per_group
+        A.graph = 1 if graph else 0
+        check(lib().cftx_program_run_batch(self.handle, byref(A)), "cftx_program_run_batch")
+        return fl.value, bus.value
+
     def free(self):
         if self.handle:
             lib().cft_program_free(self.handle)

@@ -33,7 +33,7 @@ enum Role {
     ROLE_A, ROLE_B, ROLE_C, ROLE_D,                 // streams and the result/deposits
     ROLE_IA, ROLE_IB, ROLE_IC, ROLE_ISI,            // index tables
     ROLE_BANK, ROLE_SIN, ROLE_SOUT, ROLE_COUNTS, ROLE_MASK, ROLE_LFLAGS,
-    ROLE_POOL, ROLE_TRIPS,
+    ROLE_POOL, ROLE_TRIPS, ROLE_ORDER, ROLE_REGS,
     ROLE_COUNT
 };
 
@@ -71,6 +71,7 @@ struct cft_device {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double timed_ms = 0.0;
     uint64_t timed_launches = 0;
+    uint64_t batch_launches = 0;   // kernel launches the last batched run made
 };
 
 // Bracket a compute-kernel launch with the handle's timer events (no-ops

@@ -20,8 +20,7 @@
 // This device does not publish CFT_SEQ_FEAT_AUGADD: an image that uses
 // control codes 10 or 11 is refused at load by name, as the reference
 // refuses it on any device without the bit (program.c:377-388).
-#include "cft_internal.hpp"
-#include "kernels.h"
+#include "seq_prog.hpp"
 
 #include <string.h>
 
@@ -95,12 +94,6 @@ uint32_t konst_index(const Dec &d, int k)
     if (!d.kx)
         return (uint32_t)field;
     return ((d.imm >> (8 * k)) & 0xFFu) | ((uint32_t)k9 << 8);
-}
-
-int step_of(const Dec &d)
-{
-    int s = (int)(d.imm & kStepMask);
-    return (s & 0x800) ? s - 0x1000 : s;
 }
 
 // host/src/device.c:1287-1301: a program past one of the device's published
@@ -233,26 +226,6 @@ struct Sha256 {
 };
 
 } // namespace
-
-struct cft_program {
-    cft_device *dev = nullptr;
-    int fmt = 0;
-    size_t esz = 0;
-    uint32_t max_deposits = 0, n_insns = 0, n_consts = 0, flags = 0;
-    uint32_t n_in = 0, n_out = 0, scratch_used = 0;
-    bool scratch_active = false;
-    uint32_t depth = 0;
-    int log2depth = 0;
-    std::vector<uint8_t> image;
-    std::vector<uint64_t> insns;
-    // the decoded form
-    std::vector<SeqInsn> code;
-    uint32_t nregs = 0;
-    int phys_abc[3] = {-1, -1, -1};
-    uint32_t top_repeat = 0xFFFFFFFFu;   // the first REPEAT at nesting depth 0
-    SeqInsn *d_code = nullptr;
-    uint32_t *d_consts = nullptr;         // an image's own constants, on the device
-};
 
 namespace {
 
@@ -642,9 +615,10 @@ void predecode(cft_program *p)
 
 void free_device(cft_program *p)
 {
-    if (p->d_code || p->d_consts) {
+    if (p->d_code || p->d_consts || p->gexec) {
         if (hipSetDevice(p->dev->hip_index) == hipSuccess) {
             (void)hipStreamSynchronize(p->dev->stream);
+            seq_drop_graph(p);
             if (p->d_code) (void)hipFree(p->d_code);
             if (p->d_consts) (void)hipFree(p->d_consts);
         }
@@ -1074,12 +1048,7 @@ cft_status check_round2(const cft_program *p, const cft_run_args *A, const char 
     return CFT_OK;
 }
 
-// Waves a workgroup and whether the code goes into LDS, for this program on
-// this device.
-struct Shape {
-    int waves;
-    bool code_in_lds;
-};
+} // namespace
 
 Shape shape_for(const cft_program *p, int waves_pref)
 {
@@ -1096,7 +1065,6 @@ Shape shape_for(const cft_program *p, int waves_pref)
     return {0, false};
 }
 
-} // namespace
 
 // The run itself: everything after the argument rules.  trips: per-lane trip
 // counts of the program's first top-level REPEAT (device pointer), or NULL.
@@ -1406,8 +1374,10 @@ extern "C" CFT_API cft_status cft_program_digest(cft_program *prog, const void *
     return CFT_OK;
 }
 
-// for the batched runner (cft_batch.cpp)
-uint32_t seq_top_repeat(const cft_program *p) { return p->top_repeat; }
-int seq_format(const cft_program *p) { return p->fmt; }
-size_t seq_esz(const cft_program *p) { return p->esz; }
-cft_device *seq_device(const cft_program *p) { return p->dev; }
+cft_status seq_check_bank(const cft_program *p, const void *bank, size_t bank_bytes,
+                          const char *who)
+{
+    return check_bank(p, bank, bank_bytes, who);
+}
+
+uint32_t seq_scratch_reach(const cft_program *p) { return scratch_reach(p); }

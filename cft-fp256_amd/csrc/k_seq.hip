@@ -113,100 +113,132 @@ __device__ __forceinline__ int bitlen_of(const uint32_t (&x)[NL])
     return b;
 }
 
+// LDS layout of one workgroup: code | constants | per-wave register files |
+// two words (the workgroup's flags and status).
+template <int NL> struct Lds {
+    using RT = typename Row<NL>::T;
+    uint4 *code;
+    RT *konst;
+    RT *regs;          // this lane's register 0, rows kWave apart
+    uint32_t *words;
+};
+
 template <int NL>
-__global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys_a, int phys_b,
-                                             int phys_c, uint32_t trip_pc)
+__device__ __forceinline__ Lds<NL> stage_lds(const SeqArgs &A, uint32_t nregs)
 {
     using RT = typename Row<NL>::T;
     constexpr int NQ = Row<NL>::NQ;
     extern __shared__ uint4 smem[];
-
     const uint32_t waves = blockDim.x / kWave;
     const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-
-    // ---- LDS carve-up: code | constants | per-wave register files | words
-    uint4 *s_code = smem;
+    Lds<NL> L;
+    L.code = smem;
     const uint32_t code_rows = A.code_in_lds ? A.n_insns : 0;
-    RT *s_konst = reinterpret_cast<RT *>(smem + code_rows);
+    L.konst = reinterpret_cast<RT *>(smem + code_rows);
     const uint32_t konst_rows = A.n_consts * NQ;
-    RT *s_regs = s_konst + konst_rows;
+    RT *s_regs = L.konst + konst_rows;
     const size_t wave_rows = (size_t)nregs * NQ * kWave;
-    RT *my_regs = s_regs + wave * wave_rows + lane;
-    uint32_t *s_words = reinterpret_cast<uint32_t *>(s_regs + waves * wave_rows);
-
+    L.regs = s_regs + wave * wave_rows + lane;
+    L.words = reinterpret_cast<uint32_t *>(s_regs + waves * wave_rows);
     for (uint32_t i = threadIdx.x; i < code_rows; i += blockDim.x) {
         const SeqInsn in = A.code[i];
-        s_code[i] = make_uint4(in.w, in.src, in.imm, in.aux);
+        L.code[i] = make_uint4(in.w, in.src, in.imm, in.aux);
     }
-    {
-        const RT *g = reinterpret_cast<const RT *>(A.bank);
-        for (uint32_t i = threadIdx.x; i < konst_rows; i += blockDim.x)
-            s_konst[i] = g[i];
-    }
+    const RT *g = reinterpret_cast<const RT *>(A.bank);
+    for (uint32_t i = threadIdx.x; i < konst_rows; i += blockDim.x)
+        L.konst[i] = g[i];
     if (threadIdx.x < 2)
-        s_words[threadIdx.x] = 0;
+        L.words[threadIdx.x] = 0;
     __syncthreads();
+    return L;
+}
 
-    // ---- this lane
-    const uint64_t li = A.lane0 + (uint64_t)(blockIdx.x * waves + wave) * kWave + lane;
-    const uint64_t gi = li;                        // index into the caller's arrays
-    const bool valid = li < A.n;
-    const bool keep = valid && (!A.mask || ((A.mask[li >> 3] >> (li & 7)) & 1u));
-    const uint64_t pl = li - A.lane0;              // index into the scratch pool
-    RT *my_pool = reinterpret_cast<RT *>(A.pool) + pl;
-    const size_t pstride = A.npool;
-
-    // registers: +0, then r0/r1/r2 from the streams (seq_exec.h:794-806)
-    {
-        uint32_t z[NL];
-        zero<NL>(z);
-        for (uint32_t r = 0; r < nregs; r++)
-            row_put<NL>(my_regs + (size_t)r * NQ * kWave, kWave, z);
-        if (valid) {
-            const uint32_t *src[3] = {A.a, A.b, A.c};
-            const uint32_t *idx[3] = {A.idx_a, A.idx_b, A.idx_c};
-            const int phys[3] = {phys_a, phys_b, phys_c};
-#pragma unroll
-            for (int s = 0; s < 3; s++) {
-                if (phys[s] < 0 || !src[s])
-                    continue;
-                uint32_t x[NL];
-                if (!idx[s]) {
-                    el_get<NL>(src[s], gi, x);
-                } else {
-                    const uint32_t e = idx[s][gi];
-                    if (e == 0xFFFFFFFFu) zero<NL>(x); else el_get<NL>(src[s], e, x);
-                }
-                row_put<NL>(my_regs + (size_t)phys[s] * NQ * kWave, kWave, x);
-            }
-        }
-        // the scratch a run clears, then the scratch-in block
-        if (keep && A.pool) {
-            for (uint32_t s = 0; s < A.reach; s++)
-                row_put<NL>(my_pool + (size_t)s * NQ * pstride, pstride, z);
-            for (uint32_t s = 0; A.sin && s < A.n_in; s++) {
-                uint32_t x[NL];
-                const uint64_t e = gi * A.n_in + s;
-                if (!A.idx_si) {
-                    el_get<NL>(A.sin, e, x);
-                } else {
-                    const uint32_t t = A.idx_si[e];
-                    if (t == 0xFFFFFFFFu) zero<NL>(x); else el_get<NL>(A.sin, t, x);
-                }
-                row_put<NL>(my_pool + (size_t)s * NQ * pstride, pstride, x);
-            }
-        }
-        // deposits of every kept lane start at +0 (seq_exec.h:744-760)
-        if (keep && A.dep)
-            for (uint32_t k = 0; k < A.max_dep; k++)
-                el_put<NL>(A.dep, gi * A.max_dep + k, z);
+// One workgroup's flags and status into the run's words.
+__device__ __forceinline__ void flush_words(uint32_t *s_words, uint32_t *g_words, uint32_t fl,
+                                            uint32_t status)
+{
+    if (fl)
+        atomicOr(&s_words[0], fl);
+    if (status)
+        atomicOr(&s_words[1], status);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_words[0])
+            atomicOr(&g_words[0], s_words[0]);
+        if (s_words[1])
+            atomicOr(&g_words[1], s_words[1]);
     }
+}
 
+// A lane's registers to +0 and r0/r1/r2 from the streams (element `gi`, or
+// through the index tables); then, for a lane that takes part, the scratch
+// it clears and its scratch-in block (host/src/seq_exec.h:763-812).
+template <int NL>
+__device__ __forceinline__ void lane_in(const SeqArgs &A, typename Row<NL>::T *regs,
+                                        typename Row<NL>::T *pool, size_t pstride,
+                                        uint32_t nregs, const int (&phys)[3], bool valid,
+                                        bool keep, uint64_t gi, const uint32_t *sin_row)
+{
+    constexpr int NQ = Row<NL>::NQ;
+    uint32_t z[NL];
+    zero<NL>(z);
+    for (uint32_t r = 0; r < nregs; r++)
+        row_put<NL>(regs + (size_t)r * NQ * kWave, kWave, z);
+    if (valid) {
+        const uint32_t *src[3] = {A.a, A.b, A.c};
+        const uint32_t *idx[3] = {A.idx_a, A.idx_b, A.idx_c};
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            if (phys[s] < 0 || !src[s])
+                continue;
+            uint32_t x[NL];
+            if (!idx[s]) {
+                el_get<NL>(src[s], gi, x);
+            } else {
+                const uint32_t e = idx[s][gi];
+                if (e == 0xFFFFFFFFu) zero<NL>(x); else el_get<NL>(src[s], e, x);
+            }
+            row_put<NL>(regs + (size_t)phys[s] * NQ * kWave, kWave, x);
+        }
+    }
+    if (keep && pool) {
+        for (uint32_t s = 0; s < A.reach; s++)
+            row_put<NL>(pool + (size_t)s * NQ * pstride, pstride, z);
+        for (uint32_t s = 0; sin_row && s < A.n_in; s++) {
+            uint32_t x[NL];
+            if (!A.idx_si) {
+                el_get<NL>(sin_row, s, x);
+            } else {
+                const uint32_t t = A.idx_si[gi * A.n_in + s];
+                if (t == 0xFFFFFFFFu) zero<NL>(x); else el_get<NL>(A.sin, t, x);
+            }
+            row_put<NL>(pool + (size_t)s * NQ * pstride, pstride, x);
+        }
+    }
+}
+
+// One lane through the program (host/src/seq_exec.h:412-685 for a block of
+// one lane, the wave sharing the program counter).  keep: the lane takes
+// part; trips: its count for the REPEAT at trip_pc (trip_pc past the end
+// when every REPEAT keeps the image's count).  Deposits go to element
+// gi * max_dep + count.  ORs the lane's flags and status into fl / status
+// and its lane byte into lfl; returns its deposit count.
+template <int NL>
+__device__ __forceinline__ uint32_t exec_program(const SeqArgs &A, const Lds<NL> &L,
+                                                 typename Row<NL>::T *pool, size_t pstride,
+                                                 bool keep, uint32_t my_trips, uint32_t trip_pc,
+                                                 uint64_t gi, uint32_t &fl, uint32_t &status,
+                                                 uint32_t &lfl)
+{
+    using RT = typename Row<NL>::T;
+    constexpr int NQ = Row<NL>::NQ;
+    RT *const my_regs = L.regs;
+    const RT *const s_konst = L.konst;
     bool active = keep;
     int park = 0;                 // loop level this lane has left early, 0 for none
     uint32_t left0 = 0, left1 = 0, left2 = 0, left3 = 0;
-    uint32_t count = 0, lfl = 0, fl = 0, status = 0;
     uint32_t body0 = 0, body1 = 0, body2 = 0, body3 = 0;
+    uint32_t count = 0;
     int level = 0, qdepth = 0;
 
     uint32_t pc = 0;
@@ -214,7 +246,7 @@ __global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys
         pc = uni(pc);
         uint4 in;
         if (A.code_in_lds) {
-            in = s_code[pc];
+            in = L.code[pc];
         } else {
             const SeqInsn g = A.code[pc];
             in = make_uint4(g.w, g.src, g.imm, g.aux);
@@ -255,9 +287,7 @@ __global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys
             pc = A.n_insns;
             break;
         case SEQ_REPEAT: {
-            uint32_t trips = imm;
-            if (A.trips && pc == trip_pc)
-                trips = valid ? A.trips[gi] : 0u;
+            const uint32_t trips = pc == trip_pc ? my_trips : imm;
             const bool enter = en && trips != 0;
             if (!wave_any(enter)) {
                 pc = aux + 1;                 // nobody runs the body: skip the loop
@@ -333,7 +363,7 @@ __global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys
         case SEQ_STL:
         case SEQ_LDL:
             if (en) {
-                RT *cell = my_pool + (size_t)(imm & 0x00FFFFFFu) * NQ * pstride;
+                RT *cell = pool + (size_t)(imm & 0x00FFFFFFu) * NQ * pstride;
                 uint32_t x[NL];
                 if (SEQ_W_OP(w) == SEQ_STL) {
                     row_get<NL>(my_regs + (size_t)SEQ_SRC_A(src) * NQ * kWave, kWave, x);
@@ -361,7 +391,7 @@ __global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys
                         row_put<NL>(my_regs + (size_t)SEQ_W_RD(w) * NQ * kWave, kWave, x);
                     }
                 } else {
-                    RT *cell = my_pool + (size_t)(ix[0] & (A.depth - 1u)) * NQ * pstride;
+                    RT *cell = pool + (size_t)(ix[0] & (A.depth - 1u)) * NQ * pstride;
                     if (is_st) {
                         row_get<NL>(my_regs + (size_t)SEQ_SRC_A(src) * NQ * kWave, kWave, x);
                         row_put<NL>(cell, pstride, x);
@@ -418,7 +448,42 @@ __global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys
         }
     }
 
-    // ---- a kept lane's outputs (seq_exec.h:850-920)
+    return count;
+}
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys_a, int phys_b,
+                                             int phys_c, uint32_t trip_pc)
+{
+    using RT = typename Row<NL>::T;
+    constexpr int NQ = Row<NL>::NQ;
+    const Lds<NL> L = stage_lds<NL>(A, nregs);
+    const uint32_t waves = blockDim.x / kWave;
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+
+    const uint64_t li = A.lane0 + (uint64_t)(blockIdx.x * waves + wave) * kWave + lane;
+    const uint64_t gi = li;                        // index into the caller's arrays
+    const bool valid = li < A.n;
+    const bool keep = valid && (!A.mask || ((A.mask[li >> 3] >> (li & 7)) & 1u));
+    RT *my_pool = A.pool ? reinterpret_cast<RT *>(A.pool) + (li - A.lane0) : nullptr;
+    const size_t pstride = A.npool;
+    const int phys[3] = {phys_a, phys_b, phys_c};
+
+    lane_in<NL>(A, L.regs, my_pool, pstride, nregs, phys, valid, keep, gi,
+                A.sin && !A.idx_si ? A.sin + gi * A.n_in * NL : A.sin);
+    if (keep && A.dep) {                           // deposits start at +0 (seq_exec.h:744-760)
+        uint32_t z[NL];
+        zero<NL>(z);
+        for (uint32_t k = 0; k < A.max_dep; k++)
+            el_put<NL>(A.dep, gi * A.max_dep + k, z);
+    }
+    const uint32_t my_trips = (A.trips && valid) ? A.trips[gi] : 0u;
+    uint32_t fl = 0, status = 0, lfl = 0;
+    const uint32_t count = exec_program<NL>(A, L, my_pool, pstride, keep, my_trips,
+                                            A.trips ? trip_pc : 0xFFFFFFFFu, gi, fl, status,
+                                            lfl);
+
+    // a kept lane's outputs (seq_exec.h:850-920)
     if (keep) {
         if (A.counts)
             A.counts[gi] = count;
@@ -430,17 +495,64 @@ __global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys
             el_put<NL>(A.sout, gi * A.n_out + s, x);
         }
     }
-    if (fl)
-        atomicOr(&s_words[0], fl);
-    if (status)
-        atomicOr(&s_words[1], status);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_words[0])
-            atomicOr(&A.words[0], s_words[0]);
-        if (s_words[1])
-            atomicOr(&A.words[1], s_words[1]);
+    flush_words(L.words, A.words, fl, status);
+}
+
+// The batched runner: persistent waves take tiles of lanes from a counter;
+// every lane runs its steps as a chain of segments, each a full run of the
+// program (fresh registers and scratch, its state as the scratch-in block,
+// the scratch-out block back into its state) with the segment REPEAT at
+// min(steps left, seg) trips.
+template <int NL>
+__global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32_t nregs,
+                                                   int phys_a, int phys_b, int phys_c,
+                                                   uint32_t trip_pc)
+{
+    using RT = typename Row<NL>::T;
+    constexpr int NQ = Row<NL>::NQ;
+    const Lds<NL> L = stage_lds<NL>(A, nregs);
+    const uint32_t waves = blockDim.x / kWave;
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const uint64_t slot = (uint64_t)(blockIdx.x * waves + wave) * kWave + lane;
+    RT *my_pool = A.pool ? reinterpret_cast<RT *>(A.pool) + slot : nullptr;
+    const size_t pstride = A.npool;
+    const int phys[3] = {phys_a, phys_b, phys_c};
+    uint32_t fl = 0, status = 0;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0)
+            tile = atomicAdd(B.queue, 1u);
+        tile = uni(__shfl(tile, 0));
+        if (tile >= B.ntiles)
+            break;
+        const uint64_t idx = (uint64_t)tile * B.lpw + lane;
+        const bool valid = lane < B.lpw && idx < B.n;
+        const uint32_t id = valid ? B.order[idx] : 0u;
+        uint32_t left = valid ? B.steps[id] : 0u;
+        uint32_t *st = B.state + (size_t)id * A.n_in * NL;
+        uint32_t lfl = 0;
+        while (wave_any(left > 0)) {
+            const bool keep = left > 0;
+            const uint32_t trips = left < B.seg ? left : B.seg;
+            lane_in<NL>(A, L.regs, my_pool, pstride, nregs, phys, valid, keep, id, st);
+            uint32_t seg_lfl = 0;
+            exec_program<NL>(A, L, my_pool, pstride, keep, trips, trip_pc, id, fl, status,
+                             seg_lfl);
+            if (keep) {
+                for (uint32_t s = 0; s < A.n_out; s++) {
+                    uint32_t x[NL];
+                    row_get<NL>(my_pool + (size_t)s * NQ * pstride, pstride, x);
+                    el_put<NL>(st, s, x);
+                }
+                lfl |= seg_lfl;
+                left -= trips;
+            }
+        }
+        if (valid && B.lflags)
+            B.lflags[id] = (uint8_t)lfl;
     }
+    flush_words(L.words, A.words, fl, status);
 }
 
 template <int NL>
@@ -456,6 +568,207 @@ hipError_t launch(const SeqArgs &A, uint32_t nregs, int pa, int pb, int pc, uint
     }
     hipLaunchKernelGGL((k_seq<NL>), dim3((uint32_t)blocks), dim3(waves * kWave), lds, st, A,
                        nregs, pa, pb, pc, trip_pc);
+    return hipGetLastError();
+}
+
+template <int NL>
+hipError_t launch_batch(const SeqArgs &A, const SeqBatch &B, uint32_t nregs, int pa, int pb,
+                        int pc, uint32_t trip_pc, int waves, int blocks, size_t lds,
+                        hipStream_t st)
+{
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_batch<NL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_seq_batch<NL>), dim3((uint32_t)blocks), dim3(waves * kWave), lds, st,
+                       A, B, nregs, pa, pb, pc, trip_pc);
+    return hipGetLastError();
+}
+
+template <int NL>
+int batch_occupancy(int waves, size_t lds)
+{
+    int nb = 0;
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_batch<NL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_seq_batch<NL>, waves * kWave, lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
+// ---- the plain form: one launch per instruction ---------------------------
+//
+// Registers and scratch live in HBM, [reg or slot][limb group][position],
+// position fastest; position p holds lane order[p].  The host walks the
+// instruction stream and launches k_seq_op once per instruction over the
+// positions still running, which the order (longest first) makes a prefix.
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_seq_op(SeqInsn in, typename Row<NL>::T *regs,
+                                                typename Row<NL>::T *pool, uint64_t stride,
+                                                uint64_t n, const uint32_t *bank, int strict,
+                                                uint32_t depth, int log2depth,
+                                                uint32_t *words)
+{
+    using RT = typename Row<NL>::T;
+    constexpr int NQ = Row<NL>::NQ;
+    __shared__ uint32_t s_words[2];
+    if (threadIdx.x < 2)
+        s_words[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t fl = 0, status = 0;
+    if (p < n) {
+        RT *my = regs + p;
+        RT *cellbase = pool + p;
+        const RT *konst = reinterpret_cast<const RT *>(bank);
+        const uint32_t w = in.w, src = in.src;
+        if (!SEQ_W_CTRL(w)) {
+            uint32_t xa[NL], xb[NL], xc[NL], o[NL];
+            const uint32_t ia = SEQ_SRC_A(src), ib = SEQ_SRC_B(src), ic = SEQ_SRC_C(src);
+            if (SEQ_W_KA(w)) row_get<NL>(konst + ia * NQ, 1, xa);
+            else row_get<NL>(my + (size_t)ia * NQ * stride, stride, xa);
+            if (SEQ_W_KB(w)) row_get<NL>(konst + ib * NQ, 1, xb);
+            else row_get<NL>(my + (size_t)ib * NQ * stride, stride, xb);
+            if (SEQ_W_KC(w)) row_get<NL>(konst + ic * NQ, 1, xc);
+            else row_get<NL>(my + (size_t)ic * NQ * stride, stride, xc);
+            const uint32_t op = SEQ_W_OP(w);
+            if (op <= 3)
+                fma_core<NL>((int)op, xa, xb, xc, (int)SEQ_W_RND(w), o, fl);
+            else
+                misc_op<NL>((int)op, xa, xb, xc, o, fl);
+            row_put<NL>(my + (size_t)SEQ_W_RD(w) * NQ * stride, stride, o);
+        } else {
+            const uint32_t op = SEQ_W_OP(w);
+            uint32_t x[NL];
+            if (op == SEQ_STL) {
+                row_get<NL>(my + (size_t)SEQ_SRC_A(src) * NQ * stride, stride, x);
+                row_put<NL>(cellbase + (size_t)(in.imm & 0x00FFFFFFu) * NQ * stride, stride, x);
+            } else if (op == SEQ_LDL) {
+                row_get<NL>(cellbase + (size_t)(in.imm & 0x00FFFFFFu) * NQ * stride, stride, x);
+                row_put<NL>(my + (size_t)SEQ_W_RD(w) * NQ * stride, stride, x);
+            } else if (op == SEQ_STX || op == SEQ_LDX) {
+                const uint32_t rb = SEQ_SRC_B(src);
+                RT *rbp = my + (size_t)rb * NQ * stride;
+                uint32_t ix[NL];
+                row_get<NL>(rbp, stride, ix);
+                const bool is_st = op == SEQ_STX;
+                if (strict && bitlen_of<NL>(ix) > log2depth) {
+                    status |= SEQ_ST_SCRATCH_RANGE;
+                    if (!is_st) {
+                        zero<NL>(x);
+                        row_put<NL>(my + (size_t)SEQ_W_RD(w) * NQ * stride, stride, x);
+                    }
+                } else {
+                    RT *cell = cellbase + (size_t)(ix[0] & (depth - 1u)) * NQ * stride;
+                    if (is_st) {
+                        row_get<NL>(my + (size_t)SEQ_SRC_A(src) * NQ * stride, stride, x);
+                        row_put<NL>(cell, stride, x);
+                    } else {
+                        row_get<NL>(cell, stride, x);
+                        row_put<NL>(my + (size_t)SEQ_W_RD(w) * NQ * stride, stride, x);
+                    }
+                }
+                int step = (int)(in.imm & 0xFFFu);
+                if (step & 0x800)
+                    step -= 0x1000;
+                if (step && !(!is_st && SEQ_W_RD(w) == rb)) {
+                    uint32_t cur[NL], mag[NL], zc[NL], o[NL], f = 0;
+                    row_get<NL>(rbp, stride, cur);
+                    zero<NL>(mag);
+                    zero<NL>(zc);
+                    mag[0] = (uint32_t)(step < 0 ? -step : step);
+                    misc_op<NL>(step > 0 ? 19 : 20, cur, mag, zc, o, f);
+                    row_put<NL>(rbp, stride, o);
+                }
+            }
+        }
+    }
+    if (fl)
+        atomicOr(&s_words[0], fl);
+    if (status)
+        atomicOr(&s_words[1], status);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_words[0]) atomicOr(&words[0], s_words[0]);
+        if (s_words[1]) atomicOr(&words[1], s_words[1]);
+    }
+}
+
+// A segment's start for positions [0, n): registers +0 (r0..r2 from the
+// streams at lane order[p]), scratch cleared to `reach`, the lane's state
+// as its scratch-in block.  store: the scratch-out block back instead.
+template <int NL>
+__global__ __launch_bounds__(256) void k_seq_lanes(typename Row<NL>::T *regs,
+                                                   typename Row<NL>::T *pool, uint64_t stride,
+                                                   uint64_t n, uint32_t nregs, uint32_t reach,
+                                                   uint32_t n_io, const uint32_t *order,
+                                                   uint32_t *state, const uint32_t *a,
+                                                   const uint32_t *b, const uint32_t *c,
+                                                   int pa, int pb, int pc, int store)
+{
+    constexpr int NQ = Row<NL>::NQ;
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const uint32_t id = order[p];
+    uint32_t *st = state + (size_t)id * n_io * NL;
+    uint32_t x[NL];
+    if (store) {
+        for (uint32_t s = 0; s < n_io; s++) {
+            row_get<NL>(pool + p + (size_t)s * NQ * stride, stride, x);
+            el_put<NL>(st, s, x);
+        }
+        return;
+    }
+    zero<NL>(x);
+    for (uint32_t r = 0; r < nregs; r++)
+        row_put<NL>(regs + p + (size_t)r * NQ * stride, stride, x);
+    for (uint32_t s = 0; s < reach; s++)
+        row_put<NL>(pool + p + (size_t)s * NQ * stride, stride, x);
+    const uint32_t *srcs[3] = {a, b, c};
+    const int phys[3] = {pa, pb, pc};
+    for (int k = 0; k < 3; k++)
+        if (srcs[k] && phys[k] >= 0) {
+            el_get<NL>(srcs[k], id, x);
+            row_put<NL>(regs + p + (size_t)phys[k] * NQ * stride, stride, x);
+        }
+    for (uint32_t s = 0; s < n_io; s++) {
+        el_get<NL>(st, s, x);
+        row_put<NL>(pool + p + (size_t)s * NQ * stride, stride, x);
+    }
+}
+
+template <int NL>
+hipError_t launch_op(const SeqInsn &in, void *regs, void *pool, uint64_t stride, uint64_t n,
+                     const uint32_t *bank, int strict, uint32_t depth, int log2depth,
+                     uint32_t *words, hipStream_t st)
+{
+    using RT = typename Row<NL>::T;
+    const uint32_t blocks = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL((k_seq_op<NL>), dim3(blocks), dim3(256), 0, st, in, (RT *)regs,
+                       (RT *)pool, stride, n, bank, strict, depth, log2depth, words);
+    return hipGetLastError();
+}
+
+template <int NL>
+hipError_t launch_lanes(void *regs, void *pool, uint64_t stride, uint64_t n, uint32_t nregs,
+                        uint32_t reach, uint32_t n_io, const uint32_t *order, uint32_t *state,
+                        const uint32_t *a, const uint32_t *b, const uint32_t *c, int pa, int pb,
+                        int pc, int store, hipStream_t st)
+{
+    using RT = typename Row<NL>::T;
+    const uint32_t blocks = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL((k_seq_lanes<NL>), dim3(blocks), dim3(256), 0, st, (RT *)regs,
+                       (RT *)pool, stride, n, nregs, reach, n_io, order, state, a, b, c, pa, pb,
+                       pc, store);
     return hipGetLastError();
 }
 
@@ -481,5 +794,70 @@ extern "C" hipError_t cftk_seq(int fmt, const SeqArgs *A, uint32_t nregs, int ph
     case 1: return launch<2>(*A, nregs, phys_a, phys_b, phys_c, trip_pc, waves, lds, st);
     case 2: return launch<4>(*A, nregs, phys_a, phys_b, phys_c, trip_pc, waves, lds, st);
     default: return launch<8>(*A, nregs, phys_a, phys_b, phys_c, trip_pc, waves, lds, st);
+    }
+}
+
+extern "C" hipError_t cftk_seq_batch(int fmt, const SeqArgs *A, const SeqBatch *B,
+                                     uint32_t nregs, int phys_a, int phys_b, int phys_c,
+                                     uint32_t trip_pc, int waves, int blocks, hipStream_t st)
+{
+    const size_t lds = cftk_seq_lds(fmt, A->code_in_lds ? A->n_insns : 0, A->n_consts, nregs,
+                                    waves);
+    switch (fmt) {
+    case 0: return launch_batch<1>(*A, *B, nregs, phys_a, phys_b, phys_c, trip_pc, waves,
+                                   blocks, lds, st);
+    case 1: return launch_batch<2>(*A, *B, nregs, phys_a, phys_b, phys_c, trip_pc, waves,
+                                   blocks, lds, st);
+    case 2: return launch_batch<4>(*A, *B, nregs, phys_a, phys_b, phys_c, trip_pc, waves,
+                                   blocks, lds, st);
+    default: return launch_batch<8>(*A, *B, nregs, phys_a, phys_b, phys_c, trip_pc, waves,
+                                    blocks, lds, st);
+    }
+}
+
+extern "C" int cftk_seq_batch_blocks_per_cu(int fmt, uint32_t n_insns_in_lds,
+                                            uint32_t n_consts, uint32_t nregs, int waves)
+{
+    const size_t lds = cftk_seq_lds(fmt, n_insns_in_lds, n_consts, nregs, waves);
+    switch (fmt) {
+    case 0: return batch_occupancy<1>(waves, lds);
+    case 1: return batch_occupancy<2>(waves, lds);
+    case 2: return batch_occupancy<4>(waves, lds);
+    default: return batch_occupancy<8>(waves, lds);
+    }
+}
+
+extern "C" hipError_t cftk_seq_op(int fmt, const SeqInsn *in, void *regs, void *pool,
+                                  uint64_t stride, uint64_t n, const uint32_t *bank, int strict,
+                                  uint32_t depth, int log2depth, uint32_t *words,
+                                  hipStream_t st)
+{
+    switch (fmt) {
+    case 0: return launch_op<1>(*in, regs, pool, stride, n, bank, strict, depth, log2depth,
+                                words, st);
+    case 1: return launch_op<2>(*in, regs, pool, stride, n, bank, strict, depth, log2depth,
+                                words, st);
+    case 2: return launch_op<4>(*in, regs, pool, stride, n, bank, strict, depth, log2depth,
+                                words, st);
+    default: return launch_op<8>(*in, regs, pool, stride, n, bank, strict, depth, log2depth,
+                                 words, st);
+    }
+}
+
+extern "C" hipError_t cftk_seq_lanes(int fmt, void *regs, void *pool, uint64_t stride,
+                                     uint64_t n, uint32_t nregs, uint32_t reach, uint32_t n_io,
+                                     const uint32_t *order, uint32_t *state, const uint32_t *a,
+                                     const uint32_t *b, const uint32_t *c, int pa, int pb,
+                                     int pc, int store, hipStream_t st)
+{
+    switch (fmt) {
+    case 0: return launch_lanes<1>(regs, pool, stride, n, nregs, reach, n_io, order, state, a,
+                                   b, c, pa, pb, pc, store, st);
+    case 1: return launch_lanes<2>(regs, pool, stride, n, nregs, reach, n_io, order, state, a,
+                                   b, c, pa, pb, pc, store, st);
+    case 2: return launch_lanes<4>(regs, pool, stride, n, nregs, reach, n_io, order, state, a,
+                                   b, c, pa, pb, pc, store, st);
+    default: return launch_lanes<8>(regs, pool, stride, n, nregs, reach, n_io, order, state, a,
+                                    b, c, pa, pb, pc, store, st);
     }
 }

@@ -31,6 +31,29 @@ hipError_t cftk_fill32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
 hipError_t cftk_seq(int fmt, const SeqArgs *A, uint32_t nregs, int phys_a, int phys_b,
                     int phys_c, uint32_t trip_pc, int waves, hipStream_t st);
 
+// The batched runner: `blocks` persistent workgroups of `waves` wavefronts.
+hipError_t cftk_seq_batch(int fmt, const SeqArgs *A, const SeqBatch *B, uint32_t nregs,
+                          int phys_a, int phys_b, int phys_c, uint32_t trip_pc, int waves,
+                          int blocks, hipStream_t st);
+
+// Workgroups of the batched runner one CU holds at once.
+int cftk_seq_batch_blocks_per_cu(int fmt, uint32_t n_insns_in_lds, uint32_t n_consts,
+                                 uint32_t nregs, int waves);
+
+// The plain form: one instruction over positions [0, n) of an HBM register
+// file and scratch pool (rows `stride` apart).
+hipError_t cftk_seq_op(int fmt, const SeqInsn *in, void *regs, void *pool, uint64_t stride,
+                       uint64_t n, const uint32_t *bank, int strict, uint32_t depth,
+                       int log2depth, uint32_t *words, hipStream_t st);
+
+// The plain form's segment start (store = 0) or end (store = 1) for
+// positions [0, n), position p holding lane order[p].
+hipError_t cftk_seq_lanes(int fmt, void *regs, void *pool, uint64_t stride, uint64_t n,
+                          uint32_t nregs, uint32_t reach, uint32_t n_io, const uint32_t *order,
+                          uint32_t *state, const uint32_t *a, const uint32_t *b,
+                          const uint32_t *c, int pa, int pb, int pc, int store,
+                          hipStream_t st);
+
 // LDS bytes one interpreter workgroup takes.
 size_t cftk_seq_lds(int fmt, uint32_t n_insns_in_lds, uint32_t n_consts, uint32_t nregs,
                     int waves);

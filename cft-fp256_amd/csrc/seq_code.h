@@ -72,7 +72,25 @@ struct SeqArgs {
     uint64_t lane0;                   // global index of this launch's lane 0
     uint32_t *words;                  // [0] run flags, [1] run status (ORed)
     int code_in_lds;                  // stage the code in LDS (it fits)
-    // per-lane trip count of the program's one top-level REPEAT (the batched
-    // runner's uneven segments), or NULL for the image's own count
+    // per-lane trip count of the program's first top-level REPEAT, or NULL
+    // for the image's own count
     const uint32_t *trips;
+};
+
+// The batched runner's extra arguments (k_seq_batch): lanes are handed out
+// in tiles of `lpw` to persistent waves through an atomic counter, each lane
+// runs its `steps` as a chain of segments of at most `seg` trips of the
+// segment REPEAT, its state (n_in == n_out slots) read before and written
+// after every segment.
+struct SeqBatch {
+    const uint32_t *order;            // lane ids, tile-major (longest first)
+    const uint32_t *steps;            // per lane id
+    uint32_t *state;                  // per lane id: n_in slots, lane-major
+    uint32_t *queue;                  // tile counter, 0 at launch
+    uint8_t *lflags;                  // per lane id, OR over its segments, or NULL
+    uint64_t n;                       // lanes
+    uint32_t ntiles;
+    uint32_t lpw;                     // lanes a wave takes (1..64)
+    uint32_t seg;                     // trips a segment
+    uint32_t pad;
 };

@@ -30,6 +30,48 @@ CFT_API cft_status cftx_program_check(const void *image, size_t bytes,
                                       uint32_t scratch_depth,
                                       cft_program_info *info);
 
+/* ---- batched program runs ------------------------------------------------
+ *
+ * Many instances of one program, each of uneven length, in one call: lane i
+ * runs steps[i] trips of the program's first top-level REPEAT (its segment
+ * loop) as a chain of segments of at most that REPEAT's own count - exactly
+ * the chain of cft_program_run_ex calls that carries a state from each
+ * segment's scratch-out block into the next one's scratch-in block, with the
+ * last segment's REPEAT count the remainder.  The program must declare
+ * scratch I/O with n_scratch_in == n_scratch_out (the state) and deposit
+ * nothing.  No reference counterpart: the reference chains segments one run
+ * at a time (host/tools/orbits.c, segrun.c).
+ *
+ * Modes: CFTX_BATCH_PERSISTENT runs the whole batch in one launch of
+ * persistent workgroups that take tiles of lanes (longest first) from a
+ * counter; CFTX_BATCH_PER_OP is the plain form for comparison - registers
+ * and scratch in HBM and one kernel launch per instruction, the host
+ * walking the program.  Both give the same bits. */
+#define CFTX_BATCH_PERSISTENT 0
+#define CFTX_BATCH_PER_OP     1
+
+typedef struct cftx_batch_args {
+    size_t          struct_size;
+    const void     *bank;        size_t bank_bytes;    /* BANK_EXT programs */
+    void           *state;       size_t state_bytes;   /* n x n_in elements, in place */
+    const uint32_t *steps;                             /* n entries */
+    size_t          n;
+    const void     *a, *b, *c;   /* r0..r2 of lane i: element i, or +0 where NULL */
+    uint32_t       *flags_out;
+    uint32_t       *bus_out;
+    uint8_t        *lane_flags;  size_t lane_flags_bytes;  /* n bytes: OR over a lane's segments */
+    int             mode;
+    uint32_t        lanes_per_wave;   /* 1..64 lanes a wavefront takes; 0 chooses */
+    uint32_t        waves_per_group;  /* 1, 2 or 4; 0 chooses */
+    int             graph;            /* launch through a captured HIP graph (persistent) */
+} cftx_batch_args;
+
+CFT_API cft_status cftx_program_run_batch(cft_program *prog, const cftx_batch_args *args);
+
+/* Kernel launches the handle's last cftx_program_run_batch made (2 for the
+ * persistent form: the counter reset and the runner). */
+CFT_API uint64_t cftx_batch_launches(cft_device *dev);
+
 #ifdef __cplusplus
 }
 #endif

@@ -293,3 +293,41 @@ def diff(x, y):
             idx = np.flatnonzero(x[k] != y[k])
             bad.append(f"{k} differs at {idx.size} places, first {idx[:4].tolist()}")
     return bad
+
+
+def oracle_batch(oracle, img, bank, state, steps, fmt, depth=256, threads=16):
+    """The batched runner's definition on the oracle: each lane's state
+    through its chain of segments - full segments at the image's own count,
+    then the remainder - one oracle run per distinct trip count a round.
+    Returns (state, flags, status)."""
+    esz = ESZ[fmt]
+    n = steps.size
+    slots = state.nbytes // (n * esz)
+    seg = segment_trips(img)
+    st = state.reshape(n, slots * esz).copy()
+    left = steps.astype(np.int64).copy()
+    fl = stat = 0
+    while (left > 0).any():
+        trips = np.minimum(left, seg)
+        for t in sorted(set(trips[left > 0].tolist())):
+            ids = np.flatnonzero((left > 0) & (trips == t))
+            case = {"image": P.with_trips(img, int(t)), "bank": bank,
+                    "a": np.zeros(ids.size * esz, dtype=np.uint8), "b": None, "c": None,
+                    "n": ids.size, "fmt": fmt, "scratch_in": st[ids].reshape(-1).copy(),
+                    "n_out": slots, "max_dep": 0, "mask": None}
+            o = run_oracle(oracle, case, depth=depth, threads=threads)
+            st[ids] = o["scratch_out"].reshape(ids.size, slots * esz)
+            fl |= o["flags"]
+            stat |= o["status"]
+        left -= trips
+    return st.reshape(-1), fl, stat
+
+
+def segment_trips(img):
+    n_insns, n_consts, _md, fmt, flags = struct.unpack_from("<5I", img, 8)
+    off = 32 + (0 if flags & 1 else n_consts * ESZ[fmt])
+    for i in range(n_insns):
+        w, = struct.unpack_from("<Q", img, off + 8 * i)
+        if (w >> 31) & 1 and (w & 0xFF) == P.REPEAT:
+            return w >> 32
+    raise ValueError("no REPEAT")
