@@ -1,20 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X engine on the reference's headline workload.
+"""Benchmark of the MI355X engine on the reference's headline workloads.
 
-Workload (one GPU): binary256 operands of the elementwise configuration -
-N = 2^24 elements from seed 0, exponents over the full range, 3% subnormals,
-2% specials, 5% FMA cases constructed near a rounding boundary.  One step is
-the fused multiply-add over all N elements in each of the five rounding
-modes (five cft_run calls), with the operands resident in HBM (cft_alloc
-buffers) before the timed region starts.
+Headline (value): binary256 FMA over the elementwise workload - N = 2^24
+elements from seed 0, exponents over the full range, 3% subnormals, 2%
+specials, 5% FMA cases constructed near a rounding boundary.  One step is the
+FMA over all N elements in each of the five rounding modes (five cft_run
+calls), operands resident in HBM (cft_alloc buffers) before the timed region.
+
+Beside it, in the same JSON line:
+  ops         binary256 add, mul, div and sqrt over the same operands;
+  rk4         Lorenz-63 RK4 in binary256, 65,536 trajectories of
+              log-uniform 1e3..1e5 steps, h = RN(1e-3): the batched runner in
+              one persistent launch through a HIP graph (steps/s), the plain
+              one-launch-per-instruction form on a 1/100 sample of the same
+              ensemble beside the persistent form on that sample (bit-equal),
+              and the reference's CPU program runs on a bounded sample;
+  lorenz96    Lorenz-96 RK4 in binary64 and binary256, 8,192 instances, ring
+              sizes 8..256, on a 1/100 sample of the steps;
+  cpu_baseline  the reference's CPU FMA on this host's cores, same inputs,
+              compared bit for bit.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-
-Prints one JSON line: the FMA rate (Gelem/s), the roofline of the FMA kernel
-(HIP events on the library's own stream, around the kernel alone), the
-other binary256 operations' rates, and the reference's CPU path timed on this
-host's cores on the same inputs, whose outputs are also compared with the
-GPU's bit for bit.
+    python bench.py --copy-calibration BYTES     (a plain device copy, for
+                                                  scaling HBM counters)
 """
 
 import argparse
@@ -33,6 +41,10 @@ sys.path.insert(0, os.path.join(HERE, "cft-fp256_amd"))
 METRIC = "bit-exact binary256 FMA Gelem/s; batched fp256 RK4 steps/s; p99 serve latency"
 HBM_PEAK_GBS = 8000.0
 FMA_BYTES = 128          # 3 x 32 B in, 32 B out per element
+# HBM bytes an FMA element actually moves: rocprofv3 FETCH_SIZE (x2.000) +
+# WRITE_SIZE (x1.000) at 2^22 elements, each scaled by the same counter on a
+# plain 256 MiB copy (profiles/fma_fp256_elementwise.md)
+FMA_TRAFFIC_PER_ELEM = (393746.0 + 131353.0) * 1024 / 4194304
 BYTES = {"add": 96, "mul": 96, "fma": 128, "div": 96, "sqrt": 64}
 
 
@@ -50,7 +62,224 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ops", action="store_true", help="FMA only")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--skip-rk4", action="store_true", help="no Lorenz-63/96 sections")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES, three times")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size (three times), the
+    reference point for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    x = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"bench: copied {nbytes} bytes three times")
+
+
+def ref_program_chain(img, bank, state, steps, fmt, threads):
+    """The reference's CPU program runs (oracle/_ref, its own C library) on a
+    chain of segments per lane - full segments at the image's trip count,
+    then the remainder - one software device per thread over slices of
+    lanes.  Returns (final states, flags, status, seconds) or None."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(HERE, "cft-fp256_amd"))
+    from cftamd import programs as P
+    path = os.path.join(HERE, "oracle", "_ref", "libcft_ref.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    V, U32P = C.c_void_p, C.POINTER(C.c_uint32)
+
+    class RunArgs(C.Structure):
+        _fields_ = [("struct_size", C.c_size_t), ("a", V), ("b", V), ("c", V),
+                    ("n", C.c_size_t), ("bank", V), ("bank_bytes", C.c_size_t),
+                    ("scratch_in", V), ("scratch_in_bytes", C.c_size_t),
+                    ("scratch_out", V), ("scratch_out_bytes", C.c_size_t),
+                    ("deposits", V), ("counts", U32P), ("flags_out", U32P),
+                    ("bus_out", U32P), ("idx_a", U32P), ("idx_b", U32P), ("idx_c", U32P),
+                    ("idx_a_src", C.c_size_t), ("idx_b_src", C.c_size_t),
+                    ("idx_c_src", C.c_size_t), ("idx_scratch_in", U32P),
+                    ("idx_scratch_src", C.c_size_t), ("lane_mask", V),
+                    ("lane_mask_bytes", C.c_size_t), ("lane_flags", V),
+                    ("lane_flags_bytes", C.c_size_t)]
+    lib.cft_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(V)]
+    lib.cft_close.argtypes = [V]
+    lib.cft_program_load.argtypes = [V, V, C.c_size_t, C.POINTER(V)]
+    lib.cft_program_free.argtypes = [V]
+    lib.cft_program_run_ex.argtypes = [V, C.POINTER(RunArgs)]
+    esz = {1: 8, 2: 16, 3: 32}[fmt]
+    n = steps.size
+    slots = state.nbytes // (n * esz)
+    seg = P.segment_trips(img)
+    st = state.reshape(n, slots * esz).copy()
+    res = {"flags": 0, "status": 0, "bad": 0}
+    lock = threading.Lock()
+
+    def work(lo, hi):
+        dev = V()
+        lib.cft_open(None, 0, C.byref(dev))
+        progs = {}
+        left = steps[lo:hi].astype(np.int64).copy()
+        fl = stt = 0
+        while (left > 0).any():
+            trips = np.minimum(left, seg)
+            for t in sorted(set(trips[left > 0].tolist())):
+                ids = lo + np.flatnonzero((left > 0) & (trips == t))
+                if t not in progs:
+                    im = np.frombuffer(P.with_trips(img, int(t)), dtype=np.uint8).copy()
+                    h = V()
+                    if lib.cft_program_load(dev, im.ctypes.data, im.nbytes, C.byref(h)) != 0:
+                        res["bad"] += 1
+                        return
+                    progs[t] = (h, im)
+                sin = st[ids].reshape(-1).copy()
+                sout = np.empty_like(sin)
+                a = np.zeros(ids.size * esz, dtype=np.uint8)
+                f, b = C.c_uint32(), C.c_uint32()
+                A = RunArgs()
+                A.struct_size = C.sizeof(A)
+                A.a, A.n = a.ctypes.data, ids.size
+                A.bank, A.bank_bytes = bank.ctypes.data, bank.nbytes
+                A.scratch_in, A.scratch_in_bytes = sin.ctypes.data, sin.nbytes
+                A.scratch_out, A.scratch_out_bytes = sout.ctypes.data, sout.nbytes
+                A.flags_out, A.bus_out = C.pointer(f), C.pointer(b)
+                if lib.cft_program_run_ex(progs[t][0], C.byref(A)) != 0:
+                    res["bad"] += 1
+                    return
+                st[ids] = sout.reshape(ids.size, slots * esz)
+                fl |= f.value
+                stt |= b.value
+            left -= trips
+        for h, _im in progs.values():
+            lib.cft_program_free(h)
+        lib.cft_close(dev)
+        with lock:
+            res["flags"] |= fl
+            res["status"] |= stt
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(n * t // threads, n * (t + 1) // threads))
+           for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    secs = time.perf_counter() - t0
+    if res["bad"]:
+        return None
+    return st.reshape(-1), res["flags"], res["status"], secs
+
+
+def rk4_section(dev, lib, args, threads):
+    """Lorenz-63 RK4 binary256 ensemble: persistent batched runner on the whole
+    ensemble, per-op form against it on a 1/100 sample, the reference's CPU
+    runs on a bounded sample."""
+    from cftamd import Device, Program, odes, _lib as L
+    out = {}
+    img, bank, state, steps = odes.lorenz63(65536, seed=args.seed, fmt=3)
+    total = int(steps.sum())
+    prog = Program(dev, img)
+    sbuf = dev.alloc(state.nbytes)
+    bbuf = dev.alloc(bank.nbytes)
+    bbuf.host[:] = bank
+    bbuf.to_device()
+    # warm the graph on this exact launch shape: a pass over a copy of the
+    # ensemble with one step a lane, then the timed pass over the real steps
+    sbuf.host[:] = state
+    sbuf.to_device()
+    prog.run_batch(sbuf, np.ones_like(steps), bank=bbuf, graph=True)
+    sbuf.from_device()
+    sbuf.host[:] = state
+    sbuf.to_device()
+    lib.cftx_kernel_timer(dev.handle, 1)
+    t0 = time.perf_counter()
+    fl, st = prog.run_batch(sbuf, steps, bank=bbuf, graph=True)
+    secs = time.perf_counter() - t0
+    kms, kl = ctypes.c_double(), ctypes.c_uint64()
+    lib.cftx_kernel_timer_read(dev.handle, ctypes.byref(kms), ctypes.byref(kl))
+    lib.cftx_kernel_timer(dev.handle, 0)
+    sbuf.from_device()
+    full = sbuf.host.copy()
+    out["lorenz63_fp256"] = {
+        "trajectories": int(steps.size), "total_steps": total,
+        "steps_min": int(steps.min()), "steps_max": int(steps.max()),
+        "persistent_s": round(secs, 4), "kernel_s": round(kms.value / 1e3, 4),
+        "steps_per_s": round(total / secs, 1), "alu_ops_per_s": round(53 * total / secs, 1),
+        "flags": fl, "status": st, "launches": int(lib.cftx_batch_launches(dev.handle))}
+    log(f"bench: rk4 persistent {secs:.3f} s -> {total / secs / 1e6:.1f} M steps/s")
+
+    # the plain per-op form on a 1/100 sample, beside the persistent form
+    s2 = np.maximum(1, steps // 100).astype(np.uint32)
+    ta = dev.alloc(state.nbytes)
+    ta.host[:] = state
+    ta.to_device()
+    t0 = time.perf_counter()
+    prog.run_batch(ta, s2, bank=bbuf, graph=True)
+    tp = time.perf_counter() - t0
+    tb = dev.alloc(state.nbytes)
+    tb.host[:] = state
+    tb.to_device()
+    t0 = time.perf_counter()
+    prog.run_batch(tb, s2, bank=bbuf, mode=L.BATCH_PER_OP)
+    to = time.perf_counter() - t0
+    nlaunch = int(lib.cftx_batch_launches(dev.handle))
+    ta.from_device()
+    tb.from_device()
+    out["lorenz63_fp256_sample"] = {
+        "sample": "the same ensemble at steps // 100 (at least 1)",
+        "total_steps": int(s2.sum()), "persistent_s": round(tp, 4), "per_op_s": round(to, 4),
+        "per_op_launches": nlaunch, "speedup_vs_per_op": round(to / tp, 2),
+        "bit_equal": bool(np.array_equal(ta.host, tb.host))}
+    log(f"bench: rk4 sample persistent {tp:.3f} s per-op {to:.3f} s ({nlaunch} launches)")
+    for b in (ta, tb):
+        b.free()
+
+    # the reference's CPU runs on 1,024 trajectories capped at 2,000 steps
+    if not args.skip_cpu:
+        ids = np.linspace(0, steps.size - 1, 1024).astype(np.int64)
+        cs = np.minimum(steps[ids], 2000).astype(np.uint32)
+        cstate = state.reshape(steps.size, 96)[ids].reshape(-1).copy()
+        r = ref_program_chain(img, bank, cstate, cs, 3, threads)
+        if r is not None:
+            g = cstate.copy()
+            prog.run_batch(g, cs, bank=bank)
+            out["lorenz63_fp256_cpu"] = {
+                "kind": "reference", "cores": threads, "value": round(int(cs.sum()) / r[3], 1),
+                "unit": "steps/s", "sample": f"1,024 trajectories of the ensemble (evenly "
+                f"spaced), steps capped at 2,000 ({int(cs.sum())} steps), {threads} threads, "
+                f"{r[3]:.1f} s wall", "bit_equal_gpu": bool(np.array_equal(r[0], g)) and
+                (r[1], r[2]) == (fl, st)}
+            log(f"bench: rk4 CPU {int(cs.sum()) / r[3]:.0f} steps/s on {threads} threads")
+    for b in (sbuf, bbuf):
+        b.free()
+    prog.free()
+
+    # Lorenz-96: 8,192 instances, ring 8..256, a 1/1000 sample of the steps
+    with Device("hip:0", scratch_depth=2048) as d2:
+        for fmt, name in ((1, "lorenz96_fp64"), (3, "lorenz96_fp256")):
+            groups = odes.lorenz96(8192, seed=args.seed, fmt=fmt)
+            tot = comp = 0
+            t0 = time.perf_counter()
+            flags = 0
+            for k, gimg, gbank, ids, gstate, gsteps in groups:
+                gs = np.maximum(1, gsteps // 1000).astype(np.uint32)
+                p = Program(d2, gimg)
+                f, _s = p.run_batch(gstate, gs, bank=gbank)
+                p.free()
+                flags |= f
+                tot += int(gs.sum())
+                comp += int(gs.sum()) * k
+            secs = time.perf_counter() - t0
+            out[name] = {"instances": 8192, "sample": "steps // 1000 (at least 1)",
+                         "total_steps": tot, "seconds": round(secs, 4),
+                         "steps_per_s": round(tot / secs, 1),
+                         "component_steps_per_s": round(comp / secs, 1), "flags": flags}
+            log(f"bench: {name} {tot / secs:.0f} steps/s ({secs:.2f} s)")
+    return out
 
 
 def cpu_baseline(a, b, c, gpu_out, n, threads):
@@ -117,9 +346,12 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the path shards by element index with no exchange; see DESIGN.md)",
+              f"(elements and lanes shard by index with no exchange but a flag OR)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -205,6 +437,10 @@ def main():
                          "kernel_GBs": round(BYTES[name] * n / ks / 1e9, 1)}
             log(f"bench: {name} {ops[name]}")
 
+    rk4 = None
+    if not args.skip_rk4:
+        rk4 = rk4_section(dev, lib, args, args.cpu_threads or min(16, os.cpu_count() or 1))
+
     cpu = None
     bit_exact = None
     if not args.skip_cpu:
@@ -236,13 +472,14 @@ def main():
                    "elements_per_step": 5 * n, "format": "binary256", "resident": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": round(FMA_TRAFFIC_PER_ELEM * n),
                      "kernel_ms": round(kavg_s * 1e3, 4),
                      "kernel_Gelem_s": round(n / kavg_s / 1e9, 4),
                      "bytes_per_elem": FMA_BYTES},
         "cpu_baseline": cpu,
         "bit_exact_vs_cpu": bit_exact,
         "ops": ops,
+        "rk4": rk4,
     }
     for bf in bufs:
         bf.free()

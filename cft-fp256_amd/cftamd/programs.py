@@ -325,6 +325,18 @@ GENERATORS = {"lorenz63-rk4": lorenz63, "lorenz96-rk4": lorenz96,
               "henonheiles-lf": henonheiles}
 
 
+def segment_trips(img):
+    """The trip count of an image's first REPEAT (its segment length)."""
+    n_insns, n_consts, _md, fmt, flags = struct.unpack_from("<5I", img, 8)[:5]
+    esz = FORMATS[fmt][0] // 8
+    off = 32 + (0 if flags & FLAG_BANK_EXT else n_consts * esz)
+    for i in range(n_insns):
+        w, = struct.unpack_from("<Q", img, off + 8 * i)
+        if (w >> 31) & 1 and (w & 0xFF) == REPEAT:
+            return w >> 32
+    raise ValueError("no REPEAT in this image")
+
+
 def with_trips(img, trips):
     """The same image with its first REPEAT's trip count replaced (how a
     segment of another length is built: one REPEAT, a different count)."""

@@ -323,11 +323,4 @@ def oracle_batch(oracle, img, bank, state, steps, fmt, depth=256, threads=16):
     return st.reshape(-1), fl, stat
 
 
-def segment_trips(img):
-    n_insns, n_consts, _md, fmt, flags = struct.unpack_from("<5I", img, 8)
-    off = 32 + (0 if flags & 1 else n_consts * ESZ[fmt])
-    for i in range(n_insns):
-        w, = struct.unpack_from("<Q", img, off + 8 * i)
-        if (w >> 31) & 1 and (w & 0xFF) == P.REPEAT:
-            return w >> 32
-    raise ValueError("no REPEAT")
+segment_trips = P.segment_trips
