@@ -68,6 +68,44 @@ def parse():
     return ap.parse_args()
 
 
+def ref_elem(a, b, c, n, fmt, op, threads):
+    """One elementwise op through the reference's CPU library (oracle/_ref),
+    one software device per thread: (d, flags, seconds) or None."""
+    path = os.path.join(HERE, "oracle", "_ref", "libcft_ref.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    V, U32P = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)
+    lib.cft_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(V)]
+    lib.cft_run.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
+                            ctypes.c_size_t, U32P, U32P]
+    lib.cft_close.argtypes = [V]
+    esz = {2: 16, 3: 32}[fmt]
+    d = np.empty(n * esz, dtype=np.uint8)
+    flags = [0] * threads
+
+    def work(t):
+        lo, hi = n * t // threads, n * (t + 1) // threads
+        dev = V()
+        lib.cft_open(None, 0, ctypes.byref(dev))
+        f = ctypes.c_uint32()
+        lib.cft_run(dev, op, fmt, 0, a.ctypes.data + esz * lo, b.ctypes.data + esz * lo,
+                    c.ctypes.data + esz * lo, d.ctypes.data + esz * lo, hi - lo,
+                    ctypes.byref(f), None)
+        lib.cft_close(dev)
+        flags[t] = f.value
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    fl = 0
+    for f in flags:
+        fl |= f
+    return d, fl, time.perf_counter() - t0
+
+
 def copy_calibration(nbytes):
     """A plain device-to-device copy of known size (three times), the
     reference point for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
@@ -209,7 +247,8 @@ def rk4_section(dev, lib, args, threads):
         "steps_min": int(steps.min()), "steps_max": int(steps.max()),
         "persistent_s": round(secs, 4), "kernel_s": round(kms.value / 1e3, 4),
         "steps_per_s": round(total / secs, 1), "alu_ops_per_s": round(53 * total / secs, 1),
-        "flags": fl, "status": st, "launches": int(lib.cftx_batch_launches(dev.handle))}
+        "flags": fl, "status": st, "launches": int(lib.cftx_batch_launches(dev.handle)),
+        "lanes_per_instance": int(lib.cftx_batch_width(dev.handle))}
     log(f"bench: rk4 persistent {secs:.3f} s -> {total / secs / 1e6:.1f} M steps/s")
 
     # the plain per-op form on a 1/100 sample, beside the persistent form
@@ -263,21 +302,31 @@ def rk4_section(dev, lib, args, threads):
         for fmt, name in ((1, "lorenz96_fp64"), (3, "lorenz96_fp256")):
             groups = odes.lorenz96(8192, seed=args.seed, fmt=fmt)
             tot = comp = 0
-            t0 = time.perf_counter()
             flags = 0
+            progs = []
             for k, gimg, gbank, ids, gstate, gsteps in groups:
-                gs = np.maximum(1, gsteps // 1000).astype(np.uint32)
+                # load, schedule and warm each ring size outside the timed region
                 p = Program(d2, gimg)
+                p.run_batch(gstate.copy(), np.ones_like(gsteps), bank=gbank)
+                progs.append(p)
+            widths = set()
+            t0 = time.perf_counter()
+            for p, (k, gimg, gbank, ids, gstate, gsteps) in zip(progs, groups):
+                gs = np.maximum(1, gsteps // 100).astype(np.uint32)
                 f, _s = p.run_batch(gstate, gs, bank=gbank)
-                p.free()
+                widths.add(int(lib.cftx_batch_width(d2.handle)))
                 flags |= f
                 tot += int(gs.sum())
                 comp += int(gs.sum()) * k
             secs = time.perf_counter() - t0
-            out[name] = {"instances": 8192, "sample": "steps // 1000 (at least 1)",
+            for p in progs:
+                p.free()
+            out[name] = {"instances": 8192, "sample": "steps // 100 (10..200 a lane), "
+                         "one persistent launch a ring size, 32 ring sizes",
                          "total_steps": tot, "seconds": round(secs, 4),
                          "steps_per_s": round(tot / secs, 1),
-                         "component_steps_per_s": round(comp / secs, 1), "flags": flags}
+                         "component_steps_per_s": round(comp / secs, 1), "flags": flags,
+                         "lanes_per_instance": sorted(widths)}
             log(f"bench: {name} {tot / secs:.0f} steps/s ({secs:.2f} s)")
     return out
 
@@ -436,6 +485,40 @@ def main():
                          "kernel_ms": round(ks * 1e3, 4),
                          "kernel_GBs": round(BYTES[name] * n / ks / 1e9, 1)}
             log(f"bench: {name} {ops[name]}")
+        # the binary128 configuration: FMA RNE over 2^20 elements (seed 0),
+        # checked against the reference CPU path on all of them
+        n1 = 1 << 20
+        b1 = [dev.alloc(n1 * 16) for _ in range(4)]
+        workload.elementwise(1, n1, seed=args.seed, threads=16,
+                             out=(b1[0].host, b1[1].host, b1[2].host))
+        for bf in b1[:3]:
+            bf.to_device()
+        f1 = ctypes.c_uint32()
+        lib.cft_run(dev.handle, 0, 2, 0, b1[0].data_ptr, b1[1].data_ptr, b1[2].data_ptr,
+                    b1[3].data_ptr, n1, ctypes.byref(f1), None)
+        lib.cftx_kernel_timer(dev.handle, 1)
+        t1 = time.perf_counter()
+        for _ in range(20):
+            lib.cft_run(dev.handle, 0, 2, 0, b1[0].data_ptr, b1[1].data_ptr, b1[2].data_ptr,
+                        b1[3].data_ptr, n1, ctypes.byref(f1), None)
+        dt = time.perf_counter() - t1
+        lib.cftx_kernel_timer_read(dev.handle, ctypes.byref(kms), ctypes.byref(klaunch))
+        lib.cftx_kernel_timer(dev.handle, 0)
+        ks = kms.value / 1e3 / max(1, klaunch.value)
+        b1[3].from_device()
+        ops["fma_fp128_c1"] = {"n": n1, "Gelem_s": round(20 * n1 / dt / 1e9, 4),
+                               "kernel_ms": round(ks * 1e3, 4),
+                               "kernel_Gelem_s": round(n1 / ks / 1e9, 4),
+                               "kernel_GBs": round(64 * n1 / ks / 1e9, 1), "flags": f1.value}
+        if not args.skip_cpu:
+            r = ref_elem(b1[0].host, b1[1].host, b1[2].host, n1, 2, 0, 16)
+            if r is not None:
+                ops["fma_fp128_c1"]["bit_exact_vs_cpu"] = bool(
+                    np.array_equal(r[0], b1[3].host) and r[1] == f1.value)
+                ops["fma_fp128_c1"]["cpu_Gelem_s"] = round(n1 / r[2] / 1e9, 5)
+        log(f"bench: fp128 FMA {ops['fma_fp128_c1']}")
+        for bf in b1:
+            bf.free()
 
     rk4 = None
     if not args.skip_rk4:

@@ -126,7 +126,8 @@ class CftxBatchArgs(ctypes.Structure):
                 ("flags_out", POINTER(c_uint32)), ("bus_out", POINTER(c_uint32)),
                 ("lane_flags", c_void_p), ("lane_flags_bytes", c_size_t),
                 ("mode", c_int), ("lanes_per_wave", c_uint32),
-                ("waves_per_group", c_uint32), ("graph", c_int)]
+                ("waves_per_group", c_uint32), ("graph", c_int),
+                ("lanes_per_instance", c_uint32)]
 
 
 BATCH_PERSISTENT, BATCH_PER_OP = 0, 1
@@ -184,6 +185,9 @@ def _bind(lib):
         "cftx_program_check": ([V, c_size_t, c_uint32, POINTER(CftProgramInfo)], c_int),
         "cftx_program_run_batch": ([V, POINTER(CftxBatchArgs)], c_int),
         "cftx_batch_launches": ([V], ctypes.c_uint64),
+        "cftx_batch_width": ([V], c_uint32),
+        "cftx_program_wide": ([V, c_size_t, c_uint32, c_uint32, V, c_size_t,
+                               POINTER(c_uint32)], c_int),
         "cft_lower_flags": ([V, c_uint32], None),
         "cft_raise_flags": ([V, c_uint32], None),
         "cft_test_flags": ([V, c_uint32], c_int),

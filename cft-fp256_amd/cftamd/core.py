@@ -178,7 +178,7 @@ class Program:
 
     def run_batch(self, state, steps, bank=None, a=None, b=None, c=None, lane_flags=None,
                   mode=_lib.BATCH_PERSISTENT, lanes_per_wave=0, waves_per_group=0,
-                  graph=False):
+                  graph=False, lanes_per_instance=0):
         """cftx_program_run_batch: lane i runs steps[i] trips of the segment
         loop, its state (n x slots elements) updated in place; returns
         (flags, status)."""
@@ -200,6 +200,7 @@ class Program:
         A.lanes_per_wave = lanes_per_wave
         A.waves_per_group = waves_per_group
         A.graph = 1 if graph else 0
+        A.lanes_per_instance = lanes_per_instance
         check(lib().cftx_program_run_batch(self.handle, byref(A)), "cftx_program_run_batch")
         return fl.value, bus.value
 

@@ -80,6 +80,9 @@ cft_status check(const cft_program *p, const cftx_batch_args *A)
     if (A->lanes_per_wave > 64)
         return refuse("lanes_per_wave %lu is past a wavefront's 64",
                       (unsigned long)A->lanes_per_wave);
+    if (A->lanes_per_instance > 64 || (A->lanes_per_instance & (A->lanes_per_instance - 1)))
+        return refuse("lanes_per_instance %lu is not a power of two up to 64",
+                      (unsigned long)A->lanes_per_instance);
     if (A->waves_per_group != 0 && A->waves_per_group != 1 && A->waves_per_group != 2 &&
         A->waves_per_group != 4)
         return refuse("waves_per_group %lu is not 1, 2 or 4", (unsigned long)A->waves_per_group);
@@ -171,6 +174,62 @@ struct PerOp {
     }
 };
 
+// The wide schedule of `prog` at `width`, compiled and uploaded on first use;
+// NULL when the program does not schedule there.
+WideDev *wide_at(cft_program *prog, int width)
+{
+    auto it = prog->wide.find(width);
+    if (it != prog->wide.end())
+        return it->second.failed ? nullptr : &it->second;
+    WideDev &wd = prog->wide[width];
+    const size_t ipw = 64 / width;
+    const uint32_t budget =
+        (uint32_t)std::min<size_t>(65535, (prog->dev->lds_max / 2) / (ipw * prog->esz));
+    if (!seq_wide_compile(prog, width, budget, &wd.wp)) {
+        wd.failed = true;
+        return nullptr;
+    }
+    const size_t bytes = wd.wp.recs.size() * sizeof(WideRec);
+    if (hipMalloc(&wd.d_recs, bytes) != hipSuccess ||
+        hipMemcpy(wd.d_recs, wd.wp.recs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        if (wd.d_recs)
+            (void)hipFree(wd.d_recs);
+        wd.d_recs = nullptr;
+        wd.failed = true;
+        return nullptr;
+    }
+    return &wd;
+}
+
+// Lanes an instance takes when the caller leaves it open: the width whose
+// schedule minimises (steps a segment body) x max(longest lane, total steps
+// over the instances the machine holds at once), one wavefront a SIMD.
+int choose_width(cft_program *prog, const uint32_t *steps, size_t n)
+{
+    uint64_t most = 0, total = 0;
+    for (size_t i = 0; i < n; i++) {
+        most = std::max<uint64_t>(most, steps[i]);
+        total += steps[i];
+    }
+    const double waves = (double)prog->dev->num_cus * 4;
+    const uint32_t narrow_body = prog->code[prog->top_repeat].aux - prog->top_repeat - 1;
+    double best = (double)narrow_body * std::max<double>((double)most, total / (waves * 64));
+    int pick = 1;
+    for (int w = 2; w <= 64; w *= 2) {
+        WideDev *wd = wide_at(prog, w);
+        if (!wd)
+            continue;
+        const double cap = waves * (64 / w);
+        const double cost = 1.1 * wd->wp.body_steps * std::max<double>((double)most, total / cap);
+        if (cost < best * 0.97) {
+            best = cost;
+            pick = w;
+        }
+    }
+    return pick;
+}
+
 } // namespace
 
 extern "C" CFT_API cft_status cftx_program_run_batch(cft_program *prog,
@@ -249,7 +308,126 @@ extern "C" CFT_API cft_status cftx_program_run_batch(cft_program *prog,
         return dev_fail(dev, e, kWho, "staging the state");
 
     uint64_t launches = 0;
+    int width = 1;
     if (A.mode == CFTX_BATCH_PERSISTENT) {
+        if (prog->wide_eligible < 0)
+            prog->wide_eligible = seq_wide_eligible(prog) ? 1 : 0;
+        width = (int)A.lanes_per_instance;
+        if (width > 1 && !prog->wide_eligible) {
+            cft_set_error("%s: lanes_per_instance %d needs a pure program (no deposits, "
+                          "activity, indexed scratch or flag control)", kWho, width);
+            return CFT_ERR_UNSUPPORTED;
+        }
+        if (width == 0)
+            width = prog->wide_eligible ? choose_width(prog, A.steps, n) : 1;
+    }
+    WideDev *wd = width > 1 ? wide_at(prog, width) : nullptr;
+    if (width > 1 && !wd) {
+        cft_set_error("%s: the program does not schedule over %d lanes within this "
+                      "device's LDS", kWho, width);
+        return CFT_ERR_UNSUPPORTED;
+    }
+    if (A.mode == CFTX_BATCH_PERSISTENT && wd) {
+        const WideProgram &wp = wd->wp;
+        const uint32_t ipw = 64 / (uint32_t)width;
+        const bool recs_lds = (size_t)wp.nsteps * width * 16 <= 16384;
+        int waves = A.waves_per_group ? (int)A.waves_per_group : 4;
+        while (waves > 1 && cftk_seq_wide_lds(prog->fmt, prog->n_consts,
+                                              recs_lds ? wp.nsteps : 0, width, wp.nvregs,
+                                              waves) > dev->lds_max)
+            waves /= 2;
+        if (cftk_seq_wide_lds(prog->fmt, prog->n_consts, recs_lds ? wp.nsteps : 0, width,
+                              wp.nvregs, waves) > dev->lds_max) {
+            cft_set_error("%s: %lu registers an instance over %d lanes do not fit one "
+                          "workgroup's LDS", kWho, (unsigned long)wp.nvregs, width);
+            return CFT_ERR_UNSUPPORTED;
+        }
+        const int per_cu = cftk_seq_wide_blocks_per_cu(
+            prog->fmt, prog->n_consts, recs_lds ? wp.nsteps : 0, width, wp.nvregs, waves);
+        const uint64_t ntiles = (n + ipw - 1) / ipw;
+        const uint64_t blocks = std::min<uint64_t>((ntiles + waves - 1) / waves,
+                                                   (uint64_t)dev->num_cus * per_cu);
+        const uint64_t npool = blocks * waves * ipw;
+        if (reach && (st = arena_reserve(dev, dev->stage[ROLE_POOL], npool * reach * esz,
+                                         kWho)) != CFT_OK)
+            return st;
+        SeqArgs S;
+        memset(&S, 0, sizeof S);
+        S.n_consts = prog->n_consts;
+        S.bank = (const uint32_t *)dbank;
+        S.a = (const uint32_t *)da;
+        S.b = (const uint32_t *)db;
+        S.c = (const uint32_t *)dc;
+        S.n_in = prog->n_in;
+        S.n_out = prog->n_out;
+        S.pool = reach ? (uint32_t *)dev->stage[ROLE_POOL].p : nullptr;
+        S.npool = npool;
+        S.reach = reach;
+        S.n = n;
+        S.words = dev->dwords;
+        SeqBatch B;
+        memset(&B, 0, sizeof B);
+        B.order = (const uint32_t *)dorder;
+        B.steps = (const uint32_t *)dsteps;
+        B.state = (uint32_t *)dstate;
+        B.queue = dev->dwords + 2;
+        B.lflags = (uint8_t *)dlflags;
+        B.n = n;
+        B.ntiles = (uint32_t)ntiles;
+        B.lpw = ipw;
+        B.seg = seg;
+        SeqWide Wd;
+        memset(&Wd, 0, sizeof Wd);
+        Wd.recs = reinterpret_cast<const uint4 *>(wd->d_recs);
+        Wd.nsteps = wp.nsteps;
+        Wd.width = (uint32_t)width;
+        Wd.log2w = 0;
+        while ((1u << Wd.log2w) < (uint32_t)width)
+            Wd.log2w++;
+        Wd.nvregs = wp.nvregs;
+        Wd.top_step = wp.top_step;
+        Wd.recs_in_lds = recs_lds ? 1 : 0;
+        auto record = [&]() -> hipError_t {
+            hipError_t r = cftk_fill32(dev->dwords, 0u, 3, dev->stream);
+            if (r == hipSuccess)
+                r = cftk_seq_wide(prog->fmt, &S, &B, &Wd, prog->phys_abc[0], prog->phys_abc[1],
+                                  prog->phys_abc[2], waves, (int)blocks, dev->stream);
+            return r;
+        };
+        e = timer_begin(dev);
+        if (e == hipSuccess && A.graph) {
+            const uint64_t key[24] = {(uint64_t)(uintptr_t)dorder, (uint64_t)(uintptr_t)dsteps,
+                                      (uint64_t)(uintptr_t)dstate, (uint64_t)(uintptr_t)dbank,
+                                      (uint64_t)(uintptr_t)da, (uint64_t)(uintptr_t)db,
+                                      (uint64_t)(uintptr_t)dc, (uint64_t)(uintptr_t)dlflags,
+                                      (uint64_t)(uintptr_t)S.pool, n, ipw, (uint64_t)waves,
+                                      blocks, npool, seg, reach, 2, (uint64_t)width,
+                                      (uint64_t)(uintptr_t)wd->d_recs, 0, 0, 0, 0, 0};
+            if (!prog->gexec || memcmp(key, prog->gkey, sizeof key) != 0) {
+                seq_drop_graph(prog);
+                e = hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    const hipError_t r = record();
+                    e = hipStreamEndCapture(dev->stream, &prog->graph);
+                    if (r != hipSuccess)
+                        e = r;
+                }
+                if (e == hipSuccess)
+                    e = hipGraphInstantiate(&prog->gexec, prog->graph, nullptr, nullptr, 0);
+                if (e == hipSuccess)
+                    memcpy(prog->gkey, key, sizeof key);
+            }
+            if (e == hipSuccess)
+                e = hipGraphLaunch(prog->gexec, dev->stream);
+        } else if (e == hipSuccess) {
+            e = record();
+        }
+        launches = 2;
+        if (e == hipSuccess)
+            e = timer_end(dev);
+        if (e != hipSuccess)
+            return dev_fail(dev, e, kWho, "launching the wide batched runner");
+    } else if (A.mode == CFTX_BATCH_PERSISTENT) {
         const Shape sh = shape_for(prog, A.waves_per_group ? (int)A.waves_per_group : 4);
         if (sh.waves == 0) {
             cft_set_error("%s: this program's %lu registers do not fit one workgroup's LDS",
@@ -410,6 +588,7 @@ extern "C" CFT_API cft_status cftx_program_run_batch(cft_program *prog,
     if (lbuf)
         lbuf->host_valid = false;
     dev->batch_launches = launches;
+    dev->batch_width = A.mode == CFTX_BATCH_PERSISTENT ? (uint32_t)width : 0;
     cft_emit_flags(dev, dev->hwords[0], A.flags_out);
     if (A.bus_out)
         *A.bus_out = dev->hwords[1];
@@ -419,4 +598,9 @@ extern "C" CFT_API cft_status cftx_program_run_batch(cft_program *prog,
 extern "C" CFT_API uint64_t cftx_batch_launches(cft_device *dev)
 {
     return dev ? dev->batch_launches : 0;
+}
+
+extern "C" CFT_API uint32_t cftx_batch_width(cft_device *dev)
+{
+    return dev ? dev->batch_width : 0;
 }

@@ -72,6 +72,7 @@ struct cft_device {
     double timed_ms = 0.0;
     uint64_t timed_launches = 0;
     uint64_t batch_launches = 0;   // kernel launches the last batched run made
+    uint32_t batch_width = 0;      // lanes an instance took in it
 };
 
 // Bracket a compute-kernel launch with the handle's timer events (no-ops

@@ -615,17 +615,20 @@ void predecode(cft_program *p)
 
 void free_device(cft_program *p)
 {
-    if (p->d_code || p->d_consts || p->gexec) {
+    if (p->d_code || p->d_consts || p->gexec || !p->wide.empty()) {
         if (hipSetDevice(p->dev->hip_index) == hipSuccess) {
             (void)hipStreamSynchronize(p->dev->stream);
             seq_drop_graph(p);
             if (p->d_code) (void)hipFree(p->d_code);
             if (p->d_consts) (void)hipFree(p->d_consts);
+            for (auto &kv : p->wide)
+                if (kv.second.d_recs) (void)hipFree(kv.second.d_recs);
         }
         (void)hipGetLastError();
     }
     p->d_code = nullptr;
     p->d_consts = nullptr;
+    p->wide.clear();
 }
 
 cft_status upload(cft_program *p, const char *who)
@@ -837,6 +840,51 @@ extern "C" CFT_API cft_status cftx_program_check(const void *image, size_t bytes
         memcpy(info, &v, want);
     }
     return st;
+}
+
+extern "C" CFT_API cft_status cftx_program_wide(const void *image, size_t bytes,
+                                                uint32_t scratch_depth, uint32_t width,
+                                                void *records, size_t cap, uint32_t info[6])
+{
+    cft_clear_error();
+    if (!image || (!records && cap)) {
+        cft_set_error("cftx_program_wide was given a NULL %s", !image ? "image" : "records");
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    if (width == 0 || width > 64 || (width & (width - 1))) {
+        cft_set_error("cftx_program_wide: width %lu is not a power of two up to 64",
+                      (unsigned long)width);
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    if (scratch_depth == 0)
+        scratch_depth = cftl::kDefaultDepth;
+    if ((scratch_depth & (scratch_depth - 1)) || scratch_depth > cftl::kMaxDepth) {
+        cft_set_error("cftx_program_wide: scratch_depth %lu is not a power of two up to %lu",
+                      (unsigned long)scratch_depth, (unsigned long)cftl::kMaxDepth);
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    cft_program prog;
+    cft_status st = parse_image((const uint8_t *)image, bytes, scratch_depth, &prog);
+    if (st != CFT_OK)
+        return st;
+    WideProgram wp;
+    if (!seq_wide_eligible(&prog) || !seq_wide_compile(&prog, (int)width, 65535, &wp)) {
+        cft_set_error("cftx_program_wide: the program does not run wide (it deposits, sets "
+                      "activity, indexes scratch or controls flags, or does not schedule)");
+        return CFT_ERR_UNSUPPORTED;
+    }
+    const size_t want = (size_t)wp.nsteps * width;
+    if (records)
+        memcpy(records, wp.recs.data(), std::min(cap, want) * sizeof(WideRec));
+    if (info) {
+        info[0] = wp.nsteps;
+        info[1] = wp.nvregs;
+        info[2] = wp.nregs;
+        info[3] = wp.top_step;
+        info[4] = wp.body_steps;
+        info[5] = wp.body_alu;
+    }
+    return CFT_OK;
 }
 
 extern "C" CFT_API cft_status cft_program_get_info(cft_program *prog, cft_program_info *out)

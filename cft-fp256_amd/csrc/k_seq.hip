@@ -30,6 +30,7 @@
 #include "fx_misc.hpp"
 #include "kernels.h"
 #include "seq_code.h"
+#include "seq_wide.hpp"
 
 using namespace fx;
 
@@ -555,6 +556,237 @@ __global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32
     flush_words(L.words, A.words, fl, status);
 }
 
+// ---- wide execution: an instance over W lanes --------------------------------
+//
+// A wave holds 64 / W instances; lane j of an instance executes record j of
+// each step.  Instance registers live in LDS, [register][instance][limb
+// group]; the scratch pool in HBM, [slot][limb group][instance slot], and
+// slot s of an instance is only ever touched by its lane s mod W (the
+// scheduler places loads and stores so), which keeps every scratch access in
+// one thread's program order.
+
+#define WIDE_FENCE() __builtin_amdgcn_wave_barrier()
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide Wd,
+                                                  int phys_a, int phys_b, int phys_c)
+{
+    using RT = typename Row<NL>::T;
+    constexpr int NQ = Row<NL>::NQ;
+    extern __shared__ uint4 smem[];
+    const uint32_t waves = blockDim.x / kWave;
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const uint32_t W = Wd.width, ipw = kWave >> Wd.log2w;
+    const uint32_t g = lane >> Wd.log2w, j = lane & (W - 1);
+
+    // LDS: records (when they fit) | constants | register files | lane-flag words
+    uint4 *s_recs = smem;
+    const uint32_t rec_rows = Wd.recs_in_lds ? Wd.nsteps * W : 0;
+    RT *s_konst = reinterpret_cast<RT *>(s_recs + rec_rows);
+    const uint32_t konst_rows = A.n_consts * NQ;
+    RT *s_regs = s_konst + konst_rows;
+    const size_t wave_rows = (size_t)Wd.nvregs * ipw * NQ;
+    RT *inst_regs = s_regs + wave * wave_rows + g * NQ;   // register r at + r * ipw * NQ
+    const size_t rstride = (size_t)ipw * NQ;
+    uint32_t *s_words = reinterpret_cast<uint32_t *>(s_regs + waves * wave_rows);
+    uint32_t *s_lflags = s_words + 2 + wave * kWave;        // per instance
+
+    {
+        const RT *gk = reinterpret_cast<const RT *>(A.bank);
+        for (uint32_t i = threadIdx.x; i < konst_rows; i += blockDim.x)
+            s_konst[i] = gk[i];
+        for (uint32_t i = threadIdx.x; i < rec_rows; i += blockDim.x)
+            s_recs[i] = Wd.recs[i];
+        if (threadIdx.x < 2)
+            s_words[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    const uint4 *recs = Wd.recs_in_lds ? s_recs : Wd.recs;
+    const uint64_t islot = (uint64_t)(blockIdx.x * waves + wave) * ipw + g;
+    RT *my_pool = A.pool ? reinterpret_cast<RT *>(A.pool) + islot : nullptr;
+    const size_t pstride = A.npool;
+    uint32_t fl = 0, status = 0;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0)
+            tile = atomicAdd(B.queue, 1u);
+        tile = uni(__shfl(tile, 0));
+        if (tile >= B.ntiles)
+            break;
+        const uint64_t idx = (uint64_t)tile * ipw + g;
+        const bool valid = idx < B.n;
+        const uint32_t id = valid ? B.order[idx] : 0u;
+        uint32_t left = valid ? B.steps[id] : 0u;
+        uint32_t *st = B.state + (size_t)id * A.n_in * NL;
+        uint32_t lfl = 0;
+        while (wave_any(left > 0)) {
+            const bool keep = left > 0;
+            const uint32_t trips = left < B.seg ? left : B.seg;
+            // a segment's start: registers +0, r0..r2 from the streams, the
+            // scratch cleared to reach, the state as the scratch-in block
+            {
+                uint32_t z[NL];
+                zero<NL>(z);
+                for (uint32_t r = j; r < Wd.nvregs; r += W)
+                    row_put<NL>(inst_regs + r * rstride, 1, z);
+                WIDE_FENCE();
+                if (j == 0 && valid) {
+                    const uint32_t *src[3] = {A.a, A.b, A.c};
+                    const int phys[3] = {phys_a, phys_b, phys_c};
+                    for (int k = 0; k < 3; k++)
+                        if (src[k] && phys[k] >= 0) {
+                            uint32_t x[NL];
+                            el_get<NL>(src[k], id, x);
+                            row_put<NL>(inst_regs + (uint32_t)phys[k] * rstride, 1, x);
+                        }
+                }
+                if (keep && my_pool) {
+                    for (uint32_t s = j; s < A.reach; s += W)
+                        row_put<NL>(my_pool + (size_t)s * NQ * pstride, pstride, z);
+                    for (uint32_t s = j; s < A.n_in; s += W) {
+                        uint32_t x[NL];
+                        el_get<NL>(st, s, x);
+                        row_put<NL>(my_pool + (size_t)s * NQ * pstride, pstride, x);
+                    }
+                }
+                WIDE_FENCE();
+            }
+            // the steps
+            bool active = keep;
+            int park = 0;
+            uint32_t left0 = 0, left1 = 0, left2 = 0, left3 = 0;
+            uint32_t body0 = 0, body1 = 0, body2 = 0, body3 = 0;
+            int level = 0;
+            uint32_t pc = 0;
+            while (pc < Wd.nsteps) {
+                pc = uni(pc);
+                const uint4 r0 = recs[(size_t)pc * W];
+                const uint32_t x0 = uni(r0.x);
+                const bool en = active && park == 0;
+                if (WR_KIND(x0) == WK_CTRL) {
+                    const uint32_t op = WR_OP(x0), aux = uni(r0.y), imm = uni(r0.w);
+                    if (op == SEQ_HALT) {
+                        pc = Wd.nsteps;
+                    } else if (op == SEQ_REPEAT) {
+                        const uint32_t t = pc == Wd.top_step ? trips : imm;
+                        const bool enter = en && t != 0;
+                        if (!wave_any(enter)) {
+                            pc = aux + 1;
+                            continue;
+                        }
+                        if (en && !enter)
+                            park = level + 1;
+                        switch (level) {
+                        case 0: left0 = t; body0 = pc + 1; break;
+                        case 1: left1 = t; body1 = pc + 1; break;
+                        case 2: left2 = t; body2 = pc + 1; break;
+                        default: left3 = t; body3 = pc + 1; break;
+                        }
+                        level++;
+                        pc++;
+                    } else {                                  // ENDREP
+                        bool again = false;
+                        if (en) {
+                            uint32_t l;
+                            switch (level) {
+                            case 1: l = --left0; break;
+                            case 2: l = --left1; break;
+                            case 3: l = --left2; break;
+                            default: l = --left3; break;
+                            }
+                            again = l > 0;
+                            if (!again)
+                                park = level;
+                        }
+                        if (wave_any(again)) {
+                            pc = level == 1 ? body0 : level == 2 ? body1 : level == 3 ? body2
+                                                                                   : body3;
+                        } else {
+                            if (park == level)
+                                park = 0;
+                            level--;
+                            pc++;
+                        }
+                    }
+                    continue;
+                }
+                const uint4 rec = j == 0 ? r0 : recs[(size_t)pc * W + j];
+                const uint32_t kind = WR_KIND(rec.x);
+                if (en && kind != WK_NOP) {
+                    if (kind == WK_ALU) {
+                        uint32_t xa[NL], xb[NL], xc[NL], o[NL], f = 0;
+                        const uint32_t ia = WR_A(rec.y), ib = WR_B(rec.y), ic = WR_C(rec.z);
+                        if (WR_KA(rec.x)) row_get<NL>(s_konst + ia * NQ, 1, xa);
+                        else row_get<NL>(inst_regs + ia * rstride, 1, xa);
+                        if (WR_KB(rec.x)) row_get<NL>(s_konst + ib * NQ, 1, xb);
+                        else row_get<NL>(inst_regs + ib * rstride, 1, xb);
+                        if (WR_KC(rec.x)) row_get<NL>(s_konst + ic * NQ, 1, xc);
+                        else row_get<NL>(inst_regs + ic * rstride, 1, xc);
+                        const uint32_t op = WR_OP(rec.x);
+                        if (op <= 3) {
+                            // the lanes of a bundle mix FMA, ADD, SUB and MUL: steer each
+                            // to the FMA it is defined as (host/src/softfloat.c sf_steer:
+                            // ADD a*1+c, SUB a*1+(-c), MUL a*b+(+-0, the product's sign))
+                            // so all of them run one path
+                            if (op == OP_ADD || op == OP_SUB) {
+                                enc_one<NL>(xb);
+                                if (op == OP_SUB)
+                                    xc[NL - 1] ^= 0x80000000u;
+                            } else if (op == OP_MUL) {
+                                const uint32_t sg = (xa[NL - 1] ^ xb[NL - 1]) & 0x80000000u;
+                                zero<NL>(xc);
+                                xc[NL - 1] = sg;
+                            }
+                            fma_core<NL>(OP_FMA, xa, xb, xc, (int)WR_RND(rec.x), o, f);
+                        } else {
+                            misc_op<NL>((int)op, xa, xb, xc, o, f);
+                        }
+                        row_put<NL>(inst_regs + WR_DST(rec.z) * rstride, 1, o);
+                        lfl |= f;
+                    } else if (kind == WK_LOAD) {
+                        uint32_t x[NL];
+                        row_get<NL>(my_pool + (size_t)WR_A(rec.y) * NQ * pstride, pstride, x);
+                        row_put<NL>(inst_regs + WR_DST(rec.z) * rstride, 1, x);
+                    } else if (kind == WK_STORE) {
+                        uint32_t x[NL];
+                        row_get<NL>(inst_regs + WR_A(rec.y) * rstride, 1, x);
+                        row_put<NL>(my_pool + (size_t)WR_DST(rec.z) * NQ * pstride, pstride, x);
+                    } else {                                  // MOVE
+                        uint32_t x[NL];
+                        row_get<NL>(inst_regs + WR_A(rec.y) * rstride, 1, x);
+                        row_put<NL>(inst_regs + WR_DST(rec.z) * rstride, 1, x);
+                    }
+                }
+                WIDE_FENCE();
+                pc++;
+            }
+            // the scratch-out block back into the state (each slot by its owner lane)
+            if (keep) {
+                for (uint32_t s = j; s < A.n_out; s += W) {
+                    uint32_t x[NL];
+                    row_get<NL>(my_pool + (size_t)s * NQ * pstride, pstride, x);
+                    el_put<NL>(st, s, x);
+                }
+                left -= trips;
+            }
+            WIDE_FENCE();
+        }
+        // the instance's flags: its lanes' OR
+        if (j == 0)
+            s_lflags[g] = 0;
+        WIDE_FENCE();
+        if (lfl)
+            atomicOr(&s_lflags[g], lfl);
+        WIDE_FENCE();
+        if (j == 0 && valid && B.lflags)
+            B.lflags[id] = (uint8_t)s_lflags[g];
+        fl |= lfl;
+        WIDE_FENCE();
+    }
+    flush_words(s_words, A.words, fl, status);
+}
+
 template <int NL>
 hipError_t launch(const SeqArgs &A, uint32_t nregs, int pa, int pb, int pc, uint32_t trip_pc,
                   int waves, size_t lds, hipStream_t st)
@@ -772,6 +1004,37 @@ hipError_t launch_lanes(void *regs, void *pool, uint64_t stride, uint64_t n, uin
     return hipGetLastError();
 }
 
+template <int NL>
+hipError_t launch_wide(const SeqArgs &A, const SeqBatch &B, const SeqWide &Wd, int pa, int pb,
+                       int pc, int waves, int blocks, size_t lds, hipStream_t st)
+{
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_wide<NL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_seq_wide<NL>), dim3((uint32_t)blocks), dim3(waves * kWave), lds, st,
+                       A, B, Wd, pa, pb, pc);
+    return hipGetLastError();
+}
+
+template <int NL>
+int wide_occupancy(int waves, size_t lds)
+{
+    int nb = 0;
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_wide<NL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_seq_wide<NL>, waves * kWave, lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
 } // namespace
 
 extern "C" size_t cftk_seq_lds(int fmt, uint32_t n_insns_in_lds, uint32_t n_consts,
@@ -859,5 +1122,40 @@ extern "C" hipError_t cftk_seq_lanes(int fmt, void *regs, void *pool, uint64_t s
                                    b, c, pa, pb, pc, store, st);
     default: return launch_lanes<8>(regs, pool, stride, n, nregs, reach, n_io, order, state, a,
                                     b, c, pa, pb, pc, store, st);
+    }
+}
+
+extern "C" size_t cftk_seq_wide_lds(int fmt, uint32_t n_consts, uint32_t nsteps_in_lds,
+                                    uint32_t width, uint32_t nvregs, int waves)
+{
+    const size_t row = ((size_t)1 << fmt) * 4;
+    const size_t ipw = kWave / width;
+    return (size_t)n_consts * row + (size_t)nsteps_in_lds * width * 16 +
+           (size_t)waves * nvregs * ipw * row + 8 + (size_t)waves * kWave * 4;
+}
+
+extern "C" hipError_t cftk_seq_wide(int fmt, const SeqArgs *A, const SeqBatch *B,
+                                    const SeqWide *Wd, int phys_a, int phys_b, int phys_c,
+                                    int waves, int blocks, hipStream_t st)
+{
+    const size_t lds = cftk_seq_wide_lds(fmt, A->n_consts, Wd->recs_in_lds ? Wd->nsteps : 0,
+                                         Wd->width, Wd->nvregs, waves);
+    switch (fmt) {
+    case 0: return launch_wide<1>(*A, *B, *Wd, phys_a, phys_b, phys_c, waves, blocks, lds, st);
+    case 1: return launch_wide<2>(*A, *B, *Wd, phys_a, phys_b, phys_c, waves, blocks, lds, st);
+    case 2: return launch_wide<4>(*A, *B, *Wd, phys_a, phys_b, phys_c, waves, blocks, lds, st);
+    default: return launch_wide<8>(*A, *B, *Wd, phys_a, phys_b, phys_c, waves, blocks, lds, st);
+    }
+}
+
+extern "C" int cftk_seq_wide_blocks_per_cu(int fmt, uint32_t n_consts, uint32_t nsteps_in_lds,
+                                           uint32_t width, uint32_t nvregs, int waves)
+{
+    const size_t lds = cftk_seq_wide_lds(fmt, n_consts, nsteps_in_lds, width, nvregs, waves);
+    switch (fmt) {
+    case 0: return wide_occupancy<1>(waves, lds);
+    case 1: return wide_occupancy<2>(waves, lds);
+    case 2: return wide_occupancy<4>(waves, lds);
+    default: return wide_occupancy<8>(waves, lds);
     }
 }

@@ -54,6 +54,15 @@ hipError_t cftk_seq_lanes(int fmt, void *regs, void *pool, uint64_t stride, uint
                           const uint32_t *c, int pa, int pb, int pc, int store,
                           hipStream_t st);
 
+// The wide batched runner (an instance over Wd->width lanes).
+hipError_t cftk_seq_wide(int fmt, const SeqArgs *A, const SeqBatch *B, const SeqWide *Wd,
+                         int phys_a, int phys_b, int phys_c, int waves, int blocks,
+                         hipStream_t st);
+size_t cftk_seq_wide_lds(int fmt, uint32_t n_consts, uint32_t nsteps_in_lds, uint32_t width,
+                         uint32_t nvregs, int waves);
+int cftk_seq_wide_blocks_per_cu(int fmt, uint32_t n_consts, uint32_t nsteps_in_lds,
+                                uint32_t width, uint32_t nvregs, int waves);
+
 // LDS bytes one interpreter workgroup takes.
 size_t cftk_seq_lds(int fmt, uint32_t n_insns_in_lds, uint32_t n_consts, uint32_t nregs,
                     int waves);

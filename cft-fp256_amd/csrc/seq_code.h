@@ -94,3 +94,15 @@ struct SeqBatch {
     uint32_t seg;                     // trips a segment
     uint32_t pad;
 };
+
+// The wide batched runner's program (seq_wide.hpp): `nsteps` steps of
+// `width` 16-byte records, instance registers [0, nvregs).
+struct SeqWide {
+    const uint4 *recs;
+    uint32_t nsteps;
+    uint32_t width;                   // lanes an instance takes (power of two)
+    uint32_t log2w;
+    uint32_t nvregs;
+    uint32_t top_step;                // the segment REPEAT's step
+    int recs_in_lds;
+};

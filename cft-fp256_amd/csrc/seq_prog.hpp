@@ -4,8 +4,18 @@
 
 #include <vector>
 
+#include <map>
+
 #include "cft_internal.hpp"
 #include "kernels.h"
+#include "seq_wide.hpp"
+
+// A wide schedule on the device.
+struct WideDev {
+    WideProgram wp;
+    WideRec *d_recs = nullptr;
+    bool failed = false;            // the program does not schedule at this width
+};
 
 struct cft_program {
     cft_device *dev = nullptr;
@@ -29,6 +39,9 @@ struct cft_program {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     uint64_t gkey[24] = {};
+    // wide schedules by width, built on first use
+    std::map<int, WideDev> wide;
+    int wide_eligible = -1;          // -1 not yet known
 };
 
 // Release a program's captured graph (the device current).

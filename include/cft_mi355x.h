@@ -64,6 +64,9 @@ typedef struct cftx_batch_args {
     uint32_t        lanes_per_wave;   /* 1..64 lanes a wavefront takes; 0 chooses */
     uint32_t        waves_per_group;  /* 1, 2 or 4; 0 chooses */
     int             graph;            /* launch through a captured HIP graph (persistent) */
+    uint32_t        lanes_per_instance; /* persistent form: 1 interprets an instance on one
+                                           lane; 2..64 (a power of two) spreads it over that
+                                           many lanes (a pure program only); 0 chooses */
 } cftx_batch_args;
 
 CFT_API cft_status cftx_program_run_batch(cft_program *prog, const cftx_batch_args *args);
@@ -71,6 +74,20 @@ CFT_API cft_status cftx_program_run_batch(cft_program *prog, const cftx_batch_ar
 /* Kernel launches the handle's last cftx_program_run_batch made (2 for the
  * persistent form: the counter reset and the runner). */
 CFT_API uint64_t cftx_batch_launches(cft_device *dev);
+
+/* Lanes an instance took in the handle's last persistent batched run. */
+CFT_API uint32_t cftx_batch_width(cft_device *dev);
+
+/* The wide schedule of an image without a device, for tools and tests:
+ * schedules the program for `width` lanes an instance (a power of two up to
+ * 64) at a scratch depth (0 for 256), writes up to `cap` 16-byte step
+ * records (steps x width) to `records` and the schedule's shape to info[0..5]
+ * = {steps, registers, homes, segment-REPEAT step, body steps, body ALU
+ * operations}.  CFT_ERR_UNSUPPORTED when the program is not pure (deposits,
+ * activity, indexed scratch or flag control) or does not fit. */
+CFT_API cft_status cftx_program_wide(const void *image, size_t bytes, uint32_t scratch_depth,
+                                     uint32_t width, void *records, size_t cap,
+                                     uint32_t info[6]);
 
 #ifdef __cplusplus
 }

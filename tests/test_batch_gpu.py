@@ -112,3 +112,66 @@ def test_batch_rules(device):
             p.run_batch(np.zeros(32, np.uint8), np.ones(1, np.uint32))
         assert ei.value.status == _lib.ERR_INVALID_ARGUMENT
         p.free()
+
+
+@pytest.mark.parametrize("system,fmt,k,widths", [
+    ("lorenz63-rk4", 3, None, [1, 2, 4, 8]), ("lorenz63-rk4", 1, None, [4]),
+    ("henonheiles-lf", 3, None, [2, 4]), ("lorenz96-rk4", 1, 12, [2, 16, 64]),
+    ("lorenz96-rk4", 3, 40, [8, 64])])
+def test_wide_widths_match_oracle(device, oracle, system, fmt, k, widths):
+    rng = np.random.default_rng(31)
+    if system == "lorenz63-rk4":
+        img, bank, state, steps = odes.lorenz63(200, seed=3, fmt=fmt, steps_lo=1, steps_hi=250)
+    elif system == "henonheiles-lf":
+        img, bank, state, steps = odes.henonheiles(150, seed=3, fmt=fmt, steps_lo=1,
+                                                   steps_hi=250)
+    else:
+        (_k, img, bank, _ids, state, steps), = odes.lorenz96(50, seed=3, fmt=fmt, steps_lo=1,
+                                                             steps_hi=45, ks=[k])
+    want, wfl, wst = oracle_batch(oracle, img, bank, state, steps, fmt)
+    prog = Program(device, img)
+    for w in widths:
+        for graph in (False, True):
+            st = state.copy()
+            lf = np.full(steps.size, 0xEE, dtype=np.uint8)
+            fl, status = prog.run_batch(st, steps, bank=bank, lanes_per_instance=w,
+                                        graph=graph, lane_flags=lf)
+            assert _lib.lib().cftx_batch_width(device.handle) == w
+            assert np.array_equal(st, want) and (fl, status) == (wfl, wst), (w, graph)
+            assert set(lf.tolist()) <= {0, 16}
+    st = state.copy()
+    fl, status = prog.run_batch(st, steps, bank=bank)              # the width it chooses
+    assert np.array_equal(st, want) and (fl, status) == (wfl, wst)
+    prog.free()
+
+
+def test_wide_random_pure_programs(device, oracle):
+    from test_wide import pure_program
+    from seqlib import rand_values, ESZ
+    rng = np.random.default_rng(91)
+    for it in range(24):
+        fmt = int(rng.integers(0, 4))
+        n_io = int(rng.integers(1, 6))
+        img, consts = pure_program(rng, fmt, n_io, int(rng.integers(0, 20)))
+        n = int(rng.integers(1, 300))
+        state = rand_values(rng, fmt, n * n_io)
+        steps = rng.integers(0, 7, size=n).astype(np.uint32)
+        want, wfl, wst = oracle_batch(oracle, img, None, state, steps, fmt)
+        prog = Program(device, img)
+        for w in (1, 2, 4, 16, 64):
+            st = state.copy()
+            fl, status = prog.run_batch(st, steps, lanes_per_instance=w)
+            assert np.array_equal(st, want) and (fl, status) == (wfl, wst), (it, fmt, w)
+        prog.free()
+
+
+def test_wide_refuses_impure_programs(device):
+    img = P.image(3, [P.repeat(2), P.stx(3, 4), P.endrep(), P.halt()], scratch_in=1,
+                  scratch_out=1)
+    prog = Program(device, img)
+    with pytest.raises(CftError) as ei:
+        prog.run_batch(np.zeros(32, np.uint8), np.ones(1, np.uint32), lanes_per_instance=4)
+    assert ei.value.status == _lib.ERR_UNSUPPORTED
+    st = np.zeros(32, np.uint8)
+    prog.run_batch(st, np.ones(1, np.uint32))                     # narrow: fine
+    prog.free()
