@@ -132,6 +132,40 @@ class CftxBatchArgs(ctypes.Structure):
 
 BATCH_PERSISTENT, BATCH_PER_OP = 0, 1
 
+
+class CftxBatchPart(ctypes.Structure):
+    _fields_ = [("struct_size", c_size_t), ("prog", c_void_p),
+                ("bank", c_void_p), ("bank_bytes", c_size_t),
+                ("state", c_void_p), ("state_bytes", c_size_t),
+                ("steps", POINTER(c_uint32)), ("n", c_size_t),
+                ("lane_flags", c_void_p), ("lanes_per_instance", c_uint32),
+                ("width_used", c_uint32)]
+
+JOB_ELEM, JOB_DIV, JOB_SQRT, JOB_PROGRAM = 0, 1, 2, 3
+
+
+class CftxJob(ctypes.Structure):
+    _fields_ = [("struct_size", c_size_t), ("kind", c_int), ("op", c_int), ("fmt", c_int),
+                ("rnd", c_int), ("a", c_void_p), ("b", c_void_p), ("c", c_void_p),
+                ("d", c_void_p), ("n", c_size_t), ("prog", c_void_p),
+                ("bank", c_void_p), ("bank_bytes", c_size_t),
+                ("state", c_void_p), ("state_bytes", c_size_t),
+                ("steps", POINTER(c_uint32))]
+
+
+class CftxJobResult(ctypes.Structure):
+    _fields_ = [("struct_size", c_size_t), ("status", c_int), ("flags", c_uint32),
+                ("bus", c_uint32), ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("batch_jobs", c_uint32)]
+
+
+class CftxServerStats(ctypes.Structure):
+    _fields_ = [("struct_size", c_size_t), ("jobs", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("elem_batches", ctypes.c_uint64),
+                ("program_batches", ctypes.c_uint64), ("max_batch_jobs", ctypes.c_uint64),
+                ("peak_bytes", ctypes.c_uint64), ("slices", ctypes.c_uint64),
+                ("max_slice_lanes", ctypes.c_uint64)]
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 IN_TREE = os.path.join(os.path.dirname(HERE), "lib", "libcft.so")
 
@@ -184,9 +218,20 @@ def _bind(lib):
         "cft_program_digest": ([V, V, c_size_t, V], c_int),
         "cftx_program_check": ([V, c_size_t, c_uint32, POINTER(CftProgramInfo)], c_int),
         "cftx_program_run_batch": ([V, POINTER(CftxBatchArgs)], c_int),
+        "cftx_program_run_multi": ([V, POINTER(CftxBatchPart), c_size_t, c_uint32, P, P], c_int),
+        "cftx_program_prepare": ([V, c_uint32, P], c_int),
         "cftx_batch_launches": ([V], ctypes.c_uint64),
         "cftx_batch_width": ([V], c_uint32),
-        "cftx_program_wide": ([V, c_size_t, c_uint32, c_uint32, V, c_size_t,
+        "cftx_server_open": ([V, c_int, c_size_t, POINTER(c_void_p)], c_int),
+        "cftx_server_submit": ([V, POINTER(CftxJob), POINTER(ctypes.c_uint64)], c_int),
+        "cftx_server_wait": ([V, ctypes.c_uint64, POINTER(CftxJobResult)], c_int),
+        "cftx_server_clock": ([V], ctypes.c_double),
+        "cftx_server_stats_get": ([V, POINTER(CftxServerStats)], c_int),
+        "cftx_server_close": ([V], None),
+        "cftx_server_set_batching": ([V, c_uint32], c_int),
+        "cftx_server_set_slice": ([V, ctypes.c_double], c_int),
+        "cftx_buffer_device_ptr": ([V, V, c_size_t], c_void_p),
+        "cftx_program_wide": ([V, c_size_t, c_uint32, c_uint32, c_uint32, V, c_size_t,
                                POINTER(c_uint32)], c_int),
         "cft_lower_flags": ([V, c_uint32], None),
         "cft_raise_flags": ([V, c_uint32], None),

@@ -204,6 +204,14 @@ class Program:
         check(lib().cftx_program_run_batch(self.handle, byref(A)), "cftx_program_run_batch")
         return fl.value, bus.value
 
+    def prepare(self, widths=0x7F):
+        """cftx_program_prepare: build the wide schedules named by the bit
+        mask (bit k: 2^k lanes an instance); returns the mask that built."""
+        ready = c_uint32()
+        check(lib().cftx_program_prepare(self.handle, widths, byref(ready)),
+              "cftx_program_prepare")
+        return ready.value
+
     def free(self):
         if self.handle:
             lib().cft_program_free(self.handle)
@@ -214,6 +222,31 @@ class Program:
             self.free()
         except Exception:
             pass
+
+
+def run_multi(dev, parts, waves_per_group=0):
+    """cftx_program_run_multi over parts, each a dict with prog (a Program),
+    state (updated in place), steps, and optionally bank, lane_flags and
+    lanes_per_instance.  Returns (flags, status, widths used)."""
+    arr = (_lib.CftxBatchPart * len(parts))()
+    keep = []
+    nb = lambda x: 0 if x is None else (x.nbytes if isinstance(x, np.ndarray) else x.bytes)
+    for k, p in enumerate(parts):
+        steps = np.ascontiguousarray(p["steps"], dtype=np.uint32)
+        keep.append(steps)
+        P = arr[k]
+        P.struct_size = ctypes.sizeof(P)
+        P.prog = p["prog"].handle
+        P.bank, P.bank_bytes = _ptr(p.get("bank")), nb(p.get("bank"))
+        P.state, P.state_bytes = _ptr(p["state"]), nb(p["state"])
+        P.steps = steps.ctypes.data_as(ctypes.POINTER(c_uint32))
+        P.n = steps.size
+        P.lane_flags = _ptr(p.get("lane_flags"))
+        P.lanes_per_instance = p.get("lanes_per_instance", 0)
+    fl, bus = c_uint32(), c_uint32()
+    check(lib().cftx_program_run_multi(dev.handle, arr, len(parts), waves_per_group,
+                                       byref(fl), byref(bus)), "cftx_program_run_multi")
+    return fl.value, bus.value, [arr[k].width_used for k in range(len(parts))]
 
 
 def check_image(image, scratch_depth=0):

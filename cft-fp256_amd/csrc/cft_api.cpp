@@ -441,6 +441,23 @@ static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft
     return CFT_OK;
 }
 
+cft_status device_open(int hip_index, uint32_t depth, cft_device **out)
+{
+    char name[32];
+    snprintf(name, sizeof name, "hip:%d", hip_index);
+    return open_impl(name, 0, depth, out);
+}
+
+extern "C" CFT_API void *cftx_buffer_device_ptr(cft_device *dev, const void *p, size_t bytes)
+{
+    if (!dev || !p)
+        return nullptr;
+    cft_buffer *b = find_buffer(dev, p, bytes);
+    if (!b || !b->devp)
+        return nullptr;
+    return (uint8_t *)b->devp + ((const uint8_t *)p - (const uint8_t *)b->host);
+}
+
 extern "C" CFT_API cft_status cft_open(const char *artifact, int index, cft_device **out)
 {
     cft_clear_error();

@@ -34,6 +34,39 @@ void seq_drop_graph(cft_program *p)
     memset(p->gkey, 0, sizeof p->gkey);
 }
 
+WideDev *wide_at(cft_program *prog, int width)
+{
+    auto it = prog->wide.find(width);
+    if (it != prog->wide.end())
+        return it->second.failed ? nullptr : &it->second;
+    WideDev &wd = prog->wide[width];
+    const size_t ipw = 64 / width;
+    const uint32_t budget =
+        (uint32_t)std::min<size_t>(65535, (prog->dev->lds_max / 2) / (ipw * prog->esz));
+    if (!seq_wide_compile(prog, width, budget, &wd.wp)) {
+        wd.failed = true;
+        return nullptr;
+    }
+    const size_t bytes = wd.wp.recs.size() * sizeof(WideRec);
+    if (hipMalloc(&wd.d_recs, bytes) != hipSuccess ||
+        hipMemcpy(wd.d_recs, wd.wp.recs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        if (wd.d_recs)
+            (void)hipFree(wd.d_recs);
+        wd.d_recs = nullptr;
+        wd.failed = true;
+        return nullptr;
+    }
+    return &wd;
+}
+
+bool prog_wide_eligible(cft_program *prog)
+{
+    if (prog->wide_eligible < 0)
+        prog->wide_eligible = seq_wide_eligible(prog) ? 1 : 0;
+    return prog->wide_eligible != 0;
+}
+
 namespace {
 
 const char *const kWho = "cftx_program_run_batch";
@@ -174,33 +207,6 @@ struct PerOp {
     }
 };
 
-// The wide schedule of `prog` at `width`, compiled and uploaded on first use;
-// NULL when the program does not schedule there.
-WideDev *wide_at(cft_program *prog, int width)
-{
-    auto it = prog->wide.find(width);
-    if (it != prog->wide.end())
-        return it->second.failed ? nullptr : &it->second;
-    WideDev &wd = prog->wide[width];
-    const size_t ipw = 64 / width;
-    const uint32_t budget =
-        (uint32_t)std::min<size_t>(65535, (prog->dev->lds_max / 2) / (ipw * prog->esz));
-    if (!seq_wide_compile(prog, width, budget, &wd.wp)) {
-        wd.failed = true;
-        return nullptr;
-    }
-    const size_t bytes = wd.wp.recs.size() * sizeof(WideRec);
-    if (hipMalloc(&wd.d_recs, bytes) != hipSuccess ||
-        hipMemcpy(wd.d_recs, wd.wp.recs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipGetLastError();
-        if (wd.d_recs)
-            (void)hipFree(wd.d_recs);
-        wd.d_recs = nullptr;
-        wd.failed = true;
-        return nullptr;
-    }
-    return &wd;
-}
 
 // Lanes an instance takes when the caller leaves it open: the width whose
 // schedule minimises (steps a segment body) x max(longest lane, total steps
@@ -231,6 +237,240 @@ int choose_width(cft_program *prog, const uint32_t *steps, size_t n)
 }
 
 } // namespace
+
+// The persistent batched launch on `ex`'s stream, device-side inputs bound:
+// order/steps uploaded from the host arrays (kept alive by the caller until
+// the stream passes), the width chosen or taken, the runner launched (through
+// a captured graph when asked).  Does not wait.
+cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io,
+                            const char *who, int *width_out)
+{
+    const size_t n = io.n, esz = prog->esz;
+    const uint32_t seg = prog->code[prog->top_repeat].imm;
+    const uint32_t reach = seq_scratch_reach(prog);
+    const void *dbank = io.bank, *da = io.a, *db = io.b, *dc = io.c;
+    void *dstate = io.state, *dlflags = io.lflags;
+    const void *dorder = nullptr, *dsteps = nullptr;
+    {
+        cft_status bs = bind_in(ex, ROLE_ORDER, io.order, n * 4, &dorder, who);
+        if (bs == CFT_OK)
+            bs = bind_in(ex, ROLE_TRIPS, io.steps, n * 4, &dsteps, who);
+        if (bs != CFT_OK)
+            return bs;
+    }
+    int width = 1;
+    {
+        width = (int)io.lanes_per_instance;
+        if (width > 1 && !prog_wide_eligible(prog)) {
+            cft_set_error("%s: lanes_per_instance %d needs a pure program (no deposits, "
+                          "activity, indexed scratch or flag control)", who, width);
+            return CFT_ERR_UNSUPPORTED;
+        }
+        if (width == 0)
+            width = prog->wide_eligible ? choose_width(prog, io.steps, n) : 1;
+    }
+    WideDev *wd = width > 1 ? wide_at(prog, width) : nullptr;
+    hipError_t e = hipSuccess;
+    cft_status st = CFT_OK;
+    if (width > 1 && !wd) {
+        cft_set_error("%s: the program does not schedule over %d lanes within this "
+                      "device's LDS", who, width);
+        return CFT_ERR_UNSUPPORTED;
+    }
+    if (wd) {
+        const WideProgram &wp = wd->wp;
+        const uint32_t ipw = 64 / (uint32_t)width;
+        const bool recs_lds = (size_t)wp.nsteps * width * 16 <= 16384;
+        int waves = io.waves_per_group ? (int)io.waves_per_group : 4;
+        while (waves > 1 && cftk_seq_wide_lds(prog->fmt, prog->n_consts,
+                                              recs_lds ? wp.nsteps : 0, width, wp.nvregs,
+                                              waves) > ex->lds_max)
+            waves /= 2;
+        if (cftk_seq_wide_lds(prog->fmt, prog->n_consts, recs_lds ? wp.nsteps : 0, width,
+                              wp.nvregs, waves) > ex->lds_max) {
+            cft_set_error("%s: %lu registers an instance over %d lanes do not fit one "
+                          "workgroup's LDS", who, (unsigned long)wp.nvregs, width);
+            return CFT_ERR_UNSUPPORTED;
+        }
+        const int per_cu = cftk_seq_wide_blocks_per_cu(
+            prog->fmt, prog->n_consts, recs_lds ? wp.nsteps : 0, width, wp.nvregs, waves);
+        const uint64_t ntiles = (n + ipw - 1) / ipw;
+        const uint64_t blocks = std::min<uint64_t>((ntiles + waves - 1) / waves,
+                                                   (uint64_t)ex->num_cus * per_cu);
+        const uint64_t npool = blocks * waves * ipw;
+        if (reach && (st = arena_reserve(ex, ex->stage[ROLE_POOL], npool * reach * esz,
+                                         who)) != CFT_OK)
+            return st;
+        SeqArgs S;
+        memset(&S, 0, sizeof S);
+        S.n_consts = prog->n_consts;
+        S.bank = (const uint32_t *)dbank;
+        S.a = (const uint32_t *)da;
+        S.b = (const uint32_t *)db;
+        S.c = (const uint32_t *)dc;
+        S.n_in = prog->n_in;
+        S.n_out = prog->n_out;
+        S.pool = reach ? (uint32_t *)ex->stage[ROLE_POOL].p : nullptr;
+        S.npool = npool;
+        S.reach = reach;
+        S.n = n;
+        S.words = ex->dwords;
+        SeqBatch B;
+        memset(&B, 0, sizeof B);
+        B.order = (const uint32_t *)dorder;
+        B.steps = (const uint32_t *)dsteps;
+        B.state = (uint32_t *)dstate;
+        B.queue = ex->dwords + 2;
+        B.lflags = (uint8_t *)dlflags;
+        B.n = n;
+        B.ntiles = (uint32_t)ntiles;
+        B.lpw = ipw;
+        B.seg = seg;
+        SeqWide Wd;
+        memset(&Wd, 0, sizeof Wd);
+        Wd.recs = reinterpret_cast<const uint4 *>(wd->d_recs);
+        Wd.nsteps = wp.nsteps;
+        Wd.width = (uint32_t)width;
+        Wd.log2w = 0;
+        while ((1u << Wd.log2w) < (uint32_t)width)
+            Wd.log2w++;
+        Wd.nvregs = wp.nvregs;
+        Wd.top_step = wp.top_step;
+        Wd.recs_in_lds = recs_lds ? 1 : 0;
+        auto record = [&]() -> hipError_t {
+            hipError_t r = cftk_fill32(ex->dwords, 0u, 3, ex->stream);
+            if (r == hipSuccess)
+                r = cftk_seq_wide(prog->fmt, &S, &B, &Wd, prog->phys_abc[0], prog->phys_abc[1],
+                                  prog->phys_abc[2], waves, (int)blocks, ex->stream);
+            return r;
+        };
+        e = timer_begin(ex);
+        if (e == hipSuccess && io.graph) {
+            const uint64_t key[24] = {(uint64_t)(uintptr_t)dorder, (uint64_t)(uintptr_t)dsteps,
+                                      (uint64_t)(uintptr_t)dstate, (uint64_t)(uintptr_t)dbank,
+                                      (uint64_t)(uintptr_t)da, (uint64_t)(uintptr_t)db,
+                                      (uint64_t)(uintptr_t)dc, (uint64_t)(uintptr_t)dlflags,
+                                      (uint64_t)(uintptr_t)S.pool, n, ipw, (uint64_t)waves,
+                                      blocks, npool, seg, reach, 2, (uint64_t)width,
+                                      (uint64_t)(uintptr_t)wd->d_recs, 0, 0, 0, 0, 0};
+            if (!prog->gexec || memcmp(key, prog->gkey, sizeof key) != 0) {
+                seq_drop_graph(prog);
+                e = hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    const hipError_t r = record();
+                    e = hipStreamEndCapture(ex->stream, &prog->graph);
+                    if (r != hipSuccess)
+                        e = r;
+                }
+                if (e == hipSuccess)
+                    e = hipGraphInstantiate(&prog->gexec, prog->graph, nullptr, nullptr, 0);
+                if (e == hipSuccess)
+                    memcpy(prog->gkey, key, sizeof key);
+            }
+            if (e == hipSuccess)
+                e = hipGraphLaunch(prog->gexec, ex->stream);
+        } else if (e == hipSuccess) {
+            e = record();
+        }
+        if (e == hipSuccess)
+            e = timer_end(ex);
+        if (e != hipSuccess)
+            return dev_fail(ex, e, who, "launching the wide batched runner");
+    } else {
+        const Shape sh = shape_for(prog, io.waves_per_group ? (int)io.waves_per_group : 4);
+        if (sh.waves == 0) {
+            cft_set_error("%s: this program's %lu registers do not fit one workgroup's LDS",
+                          who, (unsigned long)prog->nregs);
+            return CFT_ERR_UNSUPPORTED;
+        }
+        const int per_cu = cftk_seq_batch_blocks_per_cu(
+            prog->fmt, sh.code_in_lds ? prog->n_insns : 0, prog->n_consts, prog->nregs,
+            sh.waves);
+        const uint64_t slots = (uint64_t)ex->num_cus * per_cu * sh.waves;
+        uint32_t lpw = io.lanes_per_wave;
+        if (lpw == 0)      // fill every wave slot before packing lanes densely
+            lpw = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, (n + slots - 1) / slots));
+        const uint64_t ntiles = (n + lpw - 1) / lpw;
+        const uint64_t blocks =
+            std::min<uint64_t>((ntiles + sh.waves - 1) / sh.waves, (uint64_t)ex->num_cus * per_cu);
+        const uint64_t npool = blocks * sh.waves * 64;
+        if (reach && (st = arena_reserve(ex, ex->stage[ROLE_POOL], npool * reach * esz,
+                                         who)) != CFT_OK)
+            return st;
+        SeqArgs S;
+        memset(&S, 0, sizeof S);
+        S.code = prog->d_code;
+        S.n_insns = prog->n_insns;
+        S.n_consts = prog->n_consts;
+        S.bank = (const uint32_t *)dbank;
+        S.a = (const uint32_t *)da;
+        S.b = (const uint32_t *)db;
+        S.c = (const uint32_t *)dc;
+        S.n_in = prog->n_in;
+        S.n_out = prog->n_out;
+        S.pool = reach ? (uint32_t *)ex->stage[ROLE_POOL].p : nullptr;
+        S.npool = npool;
+        S.reach = reach;
+        S.depth = prog->depth;
+        S.log2depth = prog->log2depth;
+        S.strict = (prog->flags & CFT_PROG_FLAG_SCRATCH_STRICT) ? 1 : 0;
+        S.n = n;
+        S.words = ex->dwords;
+        S.code_in_lds = sh.code_in_lds ? 1 : 0;
+        SeqBatch B;
+        memset(&B, 0, sizeof B);
+        B.order = (const uint32_t *)dorder;
+        B.steps = (const uint32_t *)dsteps;
+        B.state = (uint32_t *)dstate;
+        B.queue = ex->dwords + 2;
+        B.lflags = (uint8_t *)dlflags;
+        B.n = n;
+        B.ntiles = (uint32_t)ntiles;
+        B.lpw = lpw;
+        B.seg = seg;
+        auto record = [&]() -> hipError_t {
+            hipError_t r = cftk_fill32(ex->dwords, 0u, 3, ex->stream);
+            if (r == hipSuccess)
+                r = cftk_seq_batch(prog->fmt, &S, &B, prog->nregs, prog->phys_abc[0],
+                                   prog->phys_abc[1], prog->phys_abc[2], prog->top_repeat,
+                                   sh.waves, (int)blocks, ex->stream);
+            return r;
+        };
+        e = timer_begin(ex);
+        if (e == hipSuccess && io.graph) {
+            const uint64_t key[24] = {(uint64_t)(uintptr_t)dorder, (uint64_t)(uintptr_t)dsteps,
+                                      (uint64_t)(uintptr_t)dstate, (uint64_t)(uintptr_t)dbank,
+                                      (uint64_t)(uintptr_t)da, (uint64_t)(uintptr_t)db,
+                                      (uint64_t)(uintptr_t)dc, (uint64_t)(uintptr_t)dlflags,
+                                      (uint64_t)(uintptr_t)S.pool, n, lpw, (uint64_t)sh.waves, blocks,
+                                      npool, seg, reach, 1, 0, 0, 0, 0, 0, 0, 0};
+            if (!prog->gexec || memcmp(key, prog->gkey, sizeof key) != 0) {
+                seq_drop_graph(prog);
+                e = hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    const hipError_t r = record();
+                    e = hipStreamEndCapture(ex->stream, &prog->graph);
+                    if (r != hipSuccess)
+                        e = r;
+                }
+                if (e == hipSuccess)
+                    e = hipGraphInstantiate(&prog->gexec, prog->graph, nullptr, nullptr, 0);
+                if (e == hipSuccess)
+                    memcpy(prog->gkey, key, sizeof key);
+            }
+            if (e == hipSuccess)
+                e = hipGraphLaunch(prog->gexec, ex->stream);
+        } else if (e == hipSuccess) {
+            e = record();
+        }
+        if (e == hipSuccess)
+            e = timer_end(ex);
+        if (e != hipSuccess)
+            return dev_fail(ex, e, who, "launching the batched runner");
+    }
+    *width_out = width;
+    return CFT_OK;
+}
 
 extern "C" CFT_API cft_status cftx_program_run_batch(cft_program *prog,
                                                      const cftx_batch_args *args)
@@ -278,8 +518,9 @@ extern "C" CFT_API cft_status cftx_program_run_batch(cft_program *prog,
 
     const void *dorder = nullptr, *dsteps = nullptr, *dbank = nullptr, *da = nullptr,
                *db = nullptr, *dc = nullptr;
-    if ((st = bind_in(dev, ROLE_ORDER, order.data(), n * 4, &dorder, kWho)) != CFT_OK ||
-        (st = bind_in(dev, ROLE_TRIPS, A.steps, n * 4, &dsteps, kWho)) != CFT_OK ||
+    if ((A.mode == CFTX_BATCH_PER_OP &&
+         ((st = bind_in(dev, ROLE_ORDER, order.data(), n * 4, &dorder, kWho)) != CFT_OK ||
+          (st = bind_in(dev, ROLE_TRIPS, A.steps, n * 4, &dsteps, kWho)) != CFT_OK)) ||
         (st = bind_in(dev, ROLE_A, A.a, n * esz, &da, kWho)) != CFT_OK ||
         (st = bind_in(dev, ROLE_B, A.b, n * esz, &db, kWho)) != CFT_OK ||
         (st = bind_in(dev, ROLE_C, A.c, n * esz, &dc, kWho)) != CFT_OK)
@@ -310,215 +551,24 @@ extern "C" CFT_API cft_status cftx_program_run_batch(cft_program *prog,
     uint64_t launches = 0;
     int width = 1;
     if (A.mode == CFTX_BATCH_PERSISTENT) {
-        if (prog->wide_eligible < 0)
-            prog->wide_eligible = seq_wide_eligible(prog) ? 1 : 0;
-        width = (int)A.lanes_per_instance;
-        if (width > 1 && !prog->wide_eligible) {
-            cft_set_error("%s: lanes_per_instance %d needs a pure program (no deposits, "
-                          "activity, indexed scratch or flag control)", kWho, width);
-            return CFT_ERR_UNSUPPORTED;
-        }
-        if (width == 0)
-            width = prog->wide_eligible ? choose_width(prog, A.steps, n) : 1;
-    }
-    WideDev *wd = width > 1 ? wide_at(prog, width) : nullptr;
-    if (width > 1 && !wd) {
-        cft_set_error("%s: the program does not schedule over %d lanes within this "
-                      "device's LDS", kWho, width);
-        return CFT_ERR_UNSUPPORTED;
-    }
-    if (A.mode == CFTX_BATCH_PERSISTENT && wd) {
-        const WideProgram &wp = wd->wp;
-        const uint32_t ipw = 64 / (uint32_t)width;
-        const bool recs_lds = (size_t)wp.nsteps * width * 16 <= 16384;
-        int waves = A.waves_per_group ? (int)A.waves_per_group : 4;
-        while (waves > 1 && cftk_seq_wide_lds(prog->fmt, prog->n_consts,
-                                              recs_lds ? wp.nsteps : 0, width, wp.nvregs,
-                                              waves) > dev->lds_max)
-            waves /= 2;
-        if (cftk_seq_wide_lds(prog->fmt, prog->n_consts, recs_lds ? wp.nsteps : 0, width,
-                              wp.nvregs, waves) > dev->lds_max) {
-            cft_set_error("%s: %lu registers an instance over %d lanes do not fit one "
-                          "workgroup's LDS", kWho, (unsigned long)wp.nvregs, width);
-            return CFT_ERR_UNSUPPORTED;
-        }
-        const int per_cu = cftk_seq_wide_blocks_per_cu(
-            prog->fmt, prog->n_consts, recs_lds ? wp.nsteps : 0, width, wp.nvregs, waves);
-        const uint64_t ntiles = (n + ipw - 1) / ipw;
-        const uint64_t blocks = std::min<uint64_t>((ntiles + waves - 1) / waves,
-                                                   (uint64_t)dev->num_cus * per_cu);
-        const uint64_t npool = blocks * waves * ipw;
-        if (reach && (st = arena_reserve(dev, dev->stage[ROLE_POOL], npool * reach * esz,
-                                         kWho)) != CFT_OK)
+        BatchIO io;
+        io.bank = dbank;
+        io.a = da;
+        io.b = db;
+        io.c = dc;
+        io.state = dstate;
+        io.lflags = (uint8_t *)dlflags;
+        io.steps = A.steps;
+        io.order = order.data();
+        io.n = n;
+        io.lanes_per_wave = A.lanes_per_wave;
+        io.waves_per_group = A.waves_per_group;
+        io.lanes_per_instance = A.lanes_per_instance;
+        io.graph = A.graph != 0;
+        st = batch_persistent(dev, prog, io, kWho, &width);
+        if (st != CFT_OK)
             return st;
-        SeqArgs S;
-        memset(&S, 0, sizeof S);
-        S.n_consts = prog->n_consts;
-        S.bank = (const uint32_t *)dbank;
-        S.a = (const uint32_t *)da;
-        S.b = (const uint32_t *)db;
-        S.c = (const uint32_t *)dc;
-        S.n_in = prog->n_in;
-        S.n_out = prog->n_out;
-        S.pool = reach ? (uint32_t *)dev->stage[ROLE_POOL].p : nullptr;
-        S.npool = npool;
-        S.reach = reach;
-        S.n = n;
-        S.words = dev->dwords;
-        SeqBatch B;
-        memset(&B, 0, sizeof B);
-        B.order = (const uint32_t *)dorder;
-        B.steps = (const uint32_t *)dsteps;
-        B.state = (uint32_t *)dstate;
-        B.queue = dev->dwords + 2;
-        B.lflags = (uint8_t *)dlflags;
-        B.n = n;
-        B.ntiles = (uint32_t)ntiles;
-        B.lpw = ipw;
-        B.seg = seg;
-        SeqWide Wd;
-        memset(&Wd, 0, sizeof Wd);
-        Wd.recs = reinterpret_cast<const uint4 *>(wd->d_recs);
-        Wd.nsteps = wp.nsteps;
-        Wd.width = (uint32_t)width;
-        Wd.log2w = 0;
-        while ((1u << Wd.log2w) < (uint32_t)width)
-            Wd.log2w++;
-        Wd.nvregs = wp.nvregs;
-        Wd.top_step = wp.top_step;
-        Wd.recs_in_lds = recs_lds ? 1 : 0;
-        auto record = [&]() -> hipError_t {
-            hipError_t r = cftk_fill32(dev->dwords, 0u, 3, dev->stream);
-            if (r == hipSuccess)
-                r = cftk_seq_wide(prog->fmt, &S, &B, &Wd, prog->phys_abc[0], prog->phys_abc[1],
-                                  prog->phys_abc[2], waves, (int)blocks, dev->stream);
-            return r;
-        };
-        e = timer_begin(dev);
-        if (e == hipSuccess && A.graph) {
-            const uint64_t key[24] = {(uint64_t)(uintptr_t)dorder, (uint64_t)(uintptr_t)dsteps,
-                                      (uint64_t)(uintptr_t)dstate, (uint64_t)(uintptr_t)dbank,
-                                      (uint64_t)(uintptr_t)da, (uint64_t)(uintptr_t)db,
-                                      (uint64_t)(uintptr_t)dc, (uint64_t)(uintptr_t)dlflags,
-                                      (uint64_t)(uintptr_t)S.pool, n, ipw, (uint64_t)waves,
-                                      blocks, npool, seg, reach, 2, (uint64_t)width,
-                                      (uint64_t)(uintptr_t)wd->d_recs, 0, 0, 0, 0, 0};
-            if (!prog->gexec || memcmp(key, prog->gkey, sizeof key) != 0) {
-                seq_drop_graph(prog);
-                e = hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal);
-                if (e == hipSuccess) {
-                    const hipError_t r = record();
-                    e = hipStreamEndCapture(dev->stream, &prog->graph);
-                    if (r != hipSuccess)
-                        e = r;
-                }
-                if (e == hipSuccess)
-                    e = hipGraphInstantiate(&prog->gexec, prog->graph, nullptr, nullptr, 0);
-                if (e == hipSuccess)
-                    memcpy(prog->gkey, key, sizeof key);
-            }
-            if (e == hipSuccess)
-                e = hipGraphLaunch(prog->gexec, dev->stream);
-        } else if (e == hipSuccess) {
-            e = record();
-        }
         launches = 2;
-        if (e == hipSuccess)
-            e = timer_end(dev);
-        if (e != hipSuccess)
-            return dev_fail(dev, e, kWho, "launching the wide batched runner");
-    } else if (A.mode == CFTX_BATCH_PERSISTENT) {
-        const Shape sh = shape_for(prog, A.waves_per_group ? (int)A.waves_per_group : 4);
-        if (sh.waves == 0) {
-            cft_set_error("%s: this program's %lu registers do not fit one workgroup's LDS",
-                          kWho, (unsigned long)prog->nregs);
-            return CFT_ERR_UNSUPPORTED;
-        }
-        const int per_cu = cftk_seq_batch_blocks_per_cu(
-            prog->fmt, sh.code_in_lds ? prog->n_insns : 0, prog->n_consts, prog->nregs,
-            sh.waves);
-        const uint64_t slots = (uint64_t)dev->num_cus * per_cu * sh.waves;
-        uint32_t lpw = A.lanes_per_wave;
-        if (lpw == 0)      // fill every wave slot before packing lanes densely
-            lpw = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, (n + slots - 1) / slots));
-        const uint64_t ntiles = (n + lpw - 1) / lpw;
-        const uint64_t blocks =
-            std::min<uint64_t>((ntiles + sh.waves - 1) / sh.waves, (uint64_t)dev->num_cus * per_cu);
-        const uint64_t npool = blocks * sh.waves * 64;
-        if (reach && (st = arena_reserve(dev, dev->stage[ROLE_POOL], npool * reach * esz,
-                                         kWho)) != CFT_OK)
-            return st;
-        SeqArgs S;
-        memset(&S, 0, sizeof S);
-        S.code = prog->d_code;
-        S.n_insns = prog->n_insns;
-        S.n_consts = prog->n_consts;
-        S.bank = (const uint32_t *)dbank;
-        S.a = (const uint32_t *)da;
-        S.b = (const uint32_t *)db;
-        S.c = (const uint32_t *)dc;
-        S.n_in = prog->n_in;
-        S.n_out = prog->n_out;
-        S.pool = reach ? (uint32_t *)dev->stage[ROLE_POOL].p : nullptr;
-        S.npool = npool;
-        S.reach = reach;
-        S.depth = prog->depth;
-        S.log2depth = prog->log2depth;
-        S.strict = (prog->flags & CFT_PROG_FLAG_SCRATCH_STRICT) ? 1 : 0;
-        S.n = n;
-        S.words = dev->dwords;
-        S.code_in_lds = sh.code_in_lds ? 1 : 0;
-        SeqBatch B;
-        memset(&B, 0, sizeof B);
-        B.order = (const uint32_t *)dorder;
-        B.steps = (const uint32_t *)dsteps;
-        B.state = (uint32_t *)dstate;
-        B.queue = dev->dwords + 2;
-        B.lflags = (uint8_t *)dlflags;
-        B.n = n;
-        B.ntiles = (uint32_t)ntiles;
-        B.lpw = lpw;
-        B.seg = seg;
-        auto record = [&]() -> hipError_t {
-            hipError_t r = cftk_fill32(dev->dwords, 0u, 3, dev->stream);
-            if (r == hipSuccess)
-                r = cftk_seq_batch(prog->fmt, &S, &B, prog->nregs, prog->phys_abc[0],
-                                   prog->phys_abc[1], prog->phys_abc[2], prog->top_repeat,
-                                   sh.waves, (int)blocks, dev->stream);
-            return r;
-        };
-        e = timer_begin(dev);
-        if (e == hipSuccess && A.graph) {
-            const uint64_t key[24] = {(uint64_t)(uintptr_t)dorder, (uint64_t)(uintptr_t)dsteps,
-                                      (uint64_t)(uintptr_t)dstate, (uint64_t)(uintptr_t)dbank,
-                                      (uint64_t)(uintptr_t)da, (uint64_t)(uintptr_t)db,
-                                      (uint64_t)(uintptr_t)dc, (uint64_t)(uintptr_t)dlflags,
-                                      (uint64_t)(uintptr_t)S.pool, n, lpw, (uint64_t)sh.waves, blocks,
-                                      npool, seg, reach, 1, 0, 0, 0, 0, 0, 0, 0};
-            if (!prog->gexec || memcmp(key, prog->gkey, sizeof key) != 0) {
-                seq_drop_graph(prog);
-                e = hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal);
-                if (e == hipSuccess) {
-                    const hipError_t r = record();
-                    e = hipStreamEndCapture(dev->stream, &prog->graph);
-                    if (r != hipSuccess)
-                        e = r;
-                }
-                if (e == hipSuccess)
-                    e = hipGraphInstantiate(&prog->gexec, prog->graph, nullptr, nullptr, 0);
-                if (e == hipSuccess)
-                    memcpy(prog->gkey, key, sizeof key);
-            }
-            if (e == hipSuccess)
-                e = hipGraphLaunch(prog->gexec, dev->stream);
-        } else if (e == hipSuccess) {
-            e = record();
-        }
-        launches = 2;
-        if (e == hipSuccess)
-            e = timer_end(dev);
-        if (e != hipSuccess)
-            return dev_fail(dev, e, kWho, "launching the batched runner");
     } else {
         // the plain form
         const uint64_t stride = n;

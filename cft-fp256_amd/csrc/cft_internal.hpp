@@ -34,6 +34,7 @@ enum Role {
     ROLE_IA, ROLE_IB, ROLE_IC, ROLE_ISI,            // index tables
     ROLE_BANK, ROLE_SIN, ROLE_SOUT, ROLE_COUNTS, ROLE_MASK, ROLE_LFLAGS,
     ROLE_POOL, ROLE_TRIPS, ROLE_ORDER, ROLE_REGS,
+    ROLE_MPROGS,                                    // the multi-program runner's tables
     ROLE_COUNT
 };
 
@@ -118,6 +119,10 @@ cft_buffer *find_buffer(cft_device *dev, const void *p, size_t bytes);
 // Before the host reads [p, p+bytes): a resident buffer whose start p points
 // into has its mirror brought up to date from the device if a run wrote it.
 cft_status host_in(cft_device *dev, const void *p, const char *who);
+
+// Another handle on GPU hip_index at scratch depth `depth` (its own stream
+// and staging arenas), for the serving runtime's workers.
+cft_status device_open(int hip_index, uint32_t depth, cft_device **out);
 
 // Kernel grid for n lanes of one element each.
 int grid_for(const cft_device *dev, size_t n);

@@ -844,7 +844,8 @@ extern "C" CFT_API cft_status cftx_program_check(const void *image, size_t bytes
 
 extern "C" CFT_API cft_status cftx_program_wide(const void *image, size_t bytes,
                                                 uint32_t scratch_depth, uint32_t width,
-                                                void *records, size_t cap, uint32_t info[6])
+                                                uint32_t max_vregs, void *records, size_t cap,
+                                                uint32_t info[6])
 {
     cft_clear_error();
     if (!image || (!records && cap)) {
@@ -868,7 +869,9 @@ extern "C" CFT_API cft_status cftx_program_wide(const void *image, size_t bytes,
     if (st != CFT_OK)
         return st;
     WideProgram wp;
-    if (!seq_wide_eligible(&prog) || !seq_wide_compile(&prog, (int)width, 65535, &wp)) {
+    if (!seq_wide_eligible(&prog) ||
+        !seq_wide_compile(&prog, (int)width, max_vregs ? std::min<uint32_t>(max_vregs, 65535)
+                                                       : 65535, &wp)) {
         cft_set_error("cftx_program_wide: the program does not run wide (it deposits, sets "
                       "activity, indexes scratch or controls flags, or does not schedule)");
         return CFT_ERR_UNSUPPORTED;

@@ -567,6 +567,202 @@ __global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32
 
 #define WIDE_FENCE() __builtin_amdgcn_wave_barrier()
 
+// One instance of a wide schedule: its records, its register file in LDS and
+// its scratch slots in HBM.
+template <int NL>
+struct WideInst {
+    using RT = typename Row<NL>::T;
+    const uint4 *recs;
+    uint32_t nsteps, W, j, top_step, nvregs, seg, reach, n_in, n_out;
+    const RT *konst;          // constant k at konst + k * NQ
+    RT *regs;                 // register r at regs + r * rstride
+    size_t rstride;
+    RT *pool;                 // scratch slot s at pool + s * NQ * pstride
+    size_t pstride;
+};
+
+// Every segment of one instance, its W lanes together: `left` trips of the
+// segment loop as segments of `seg` trips and a last one of the remainder,
+// each a fresh run (registers +0, r0..r2 from the streams, scratch cleared
+// to reach, the state as the scratch-in block, the scratch-out block back
+// into the state).  Returns this lane's flags.
+template <int NL>
+__device__ __forceinline__ uint32_t wide_run(const WideInst<NL> &I, const bool valid,
+                                             uint32_t left, uint32_t *st,
+                                             const uint32_t *const src[3], const int phys[3],
+                                             const uint32_t id)
+{
+    constexpr int NQ = Row<NL>::NQ;
+    const uint32_t W = I.W, j = I.j;
+    const size_t rstride = I.rstride, pstride = I.pstride;
+    uint32_t lfl = 0;
+    while (wave_any(left > 0)) {
+        const bool keep = left > 0;
+        const uint32_t trips = left < I.seg ? left : I.seg;
+        {
+            uint32_t z[NL];
+            zero<NL>(z);
+            for (uint32_t r = j; r < I.nvregs; r += W)
+                row_put<NL>(I.regs + r * rstride, 1, z);
+            WIDE_FENCE();
+            if (j == 0 && valid) {
+                for (int k = 0; k < 3; k++)
+                    if (src[k] && phys[k] >= 0) {
+                        uint32_t x[NL];
+                        el_get<NL>(src[k], id, x);
+                        row_put<NL>(I.regs + (uint32_t)phys[k] * rstride, 1, x);
+                    }
+            }
+            if (keep && I.pool) {
+                for (uint32_t s = j; s < I.reach; s += W)
+                    row_put<NL>(I.pool + (size_t)s * NQ * pstride, pstride, z);
+                for (uint32_t s = j; s < I.n_in; s += W) {
+                    uint32_t x[NL];
+                    el_get<NL>(st, s, x);
+                    row_put<NL>(I.pool + (size_t)s * NQ * pstride, pstride, x);
+                }
+            }
+            WIDE_FENCE();
+        }
+        // the steps
+        bool active = keep;
+        int park = 0;
+        uint32_t left0 = 0, left1 = 0, left2 = 0, left3 = 0;
+        uint32_t body0 = 0, body1 = 0, body2 = 0, body3 = 0;
+        int level = 0;
+        uint32_t pc = 0;
+        while (pc < I.nsteps) {
+            pc = uni(pc);
+            const uint4 r0 = I.recs[(size_t)pc * W];
+            const uint32_t x0 = uni(r0.x);
+            const bool en = active && park == 0;
+            if (WR_KIND(x0) == WK_CTRL) {
+                const uint32_t op = WR_OP(x0), aux = uni(r0.y), imm = uni(r0.w);
+                if (op == SEQ_HALT) {
+                    pc = I.nsteps;
+                } else if (op == SEQ_REPEAT) {
+                    const uint32_t t = pc == I.top_step ? trips : imm;
+                    const bool enter = en && t != 0;
+                    if (!wave_any(enter)) {
+                        pc = aux + 1;
+                        continue;
+                    }
+                    if (en && !enter)
+                        park = level + 1;
+                    switch (level) {
+                    case 0: left0 = t; body0 = pc + 1; break;
+                    case 1: left1 = t; body1 = pc + 1; break;
+                    case 2: left2 = t; body2 = pc + 1; break;
+                    default: left3 = t; body3 = pc + 1; break;
+                    }
+                    level++;
+                    pc++;
+                } else {                                  // ENDREP
+                    bool again = false;
+                    if (en) {
+                        uint32_t l;
+                        switch (level) {
+                        case 1: l = --left0; break;
+                        case 2: l = --left1; break;
+                        case 3: l = --left2; break;
+                        default: l = --left3; break;
+                        }
+                        again = l > 0;
+                        if (!again)
+                            park = level;
+                    }
+                    if (wave_any(again)) {
+                        pc = level == 1 ? body0 : level == 2 ? body1 : level == 3 ? body2
+                                                                               : body3;
+                    } else {
+                        if (park == level)
+                            park = 0;
+                        level--;
+                        pc++;
+                    }
+                }
+                continue;
+            }
+            const uint4 rec = j == 0 ? r0 : I.recs[(size_t)pc * W + j];
+            const uint32_t kind = WR_KIND(rec.x);
+            if (en && kind != WK_NOP) {
+                if (kind == WK_ALU) {
+                    uint32_t xa[NL], xb[NL], xc[NL], o[NL], f = 0;
+                    const uint32_t ia = WR_A(rec.y), ib = WR_B(rec.y), ic = WR_C(rec.z);
+                    if (WR_KA(rec.x)) row_get<NL>(I.konst + ia * NQ, 1, xa);
+                    else row_get<NL>(I.regs + ia * rstride, 1, xa);
+                    if (WR_KB(rec.x)) row_get<NL>(I.konst + ib * NQ, 1, xb);
+                    else row_get<NL>(I.regs + ib * rstride, 1, xb);
+                    if (WR_KC(rec.x)) row_get<NL>(I.konst + ic * NQ, 1, xc);
+                    else row_get<NL>(I.regs + ic * rstride, 1, xc);
+                    const uint32_t op = WR_OP(rec.x);
+                    if (op <= 3) {
+                        // the lanes of a bundle mix FMA, ADD, SUB and MUL: steer each
+                        // to the FMA it is defined as (host/src/softfloat.c sf_steer:
+                        // ADD a*1+c, SUB a*1+(-c), MUL a*b+(+-0, the product's sign))
+                        // so all of them run one path
+                        if (op == OP_ADD || op == OP_SUB) {
+                            enc_one<NL>(xb);
+                            if (op == OP_SUB)
+                                xc[NL - 1] ^= 0x80000000u;
+                        } else if (op == OP_MUL) {
+                            const uint32_t sg = (xa[NL - 1] ^ xb[NL - 1]) & 0x80000000u;
+                            zero<NL>(xc);
+                            xc[NL - 1] = sg;
+                        }
+                        fma_core<NL>(OP_FMA, xa, xb, xc, (int)WR_RND(rec.x), o, f);
+                    } else {
+                        misc_op<NL>((int)op, xa, xb, xc, o, f);
+                    }
+                    row_put<NL>(I.regs + WR_DST(rec.z) * rstride, 1, o);
+                    lfl |= f;
+                } else if (kind == WK_LOAD) {
+                    uint32_t x[NL];
+                    row_get<NL>(I.pool + (size_t)WR_A(rec.y) * NQ * pstride, pstride, x);
+                    row_put<NL>(I.regs + WR_DST(rec.z) * rstride, 1, x);
+                } else if (kind == WK_STORE) {
+                    uint32_t x[NL];
+                    row_get<NL>(I.regs + WR_A(rec.y) * rstride, 1, x);
+                    row_put<NL>(I.pool + (size_t)WR_DST(rec.z) * NQ * pstride, pstride, x);
+                } else {                                  // MOVE
+                    uint32_t x[NL];
+                    row_get<NL>(I.regs + WR_A(rec.y) * rstride, 1, x);
+                    row_put<NL>(I.regs + WR_DST(rec.z) * rstride, 1, x);
+                }
+            }
+            WIDE_FENCE();
+            pc++;
+        }
+        // the scratch-out block back into the state (each slot by its owner lane)
+        if (keep) {
+            for (uint32_t s = j; s < I.n_out; s += W) {
+                uint32_t x[NL];
+                row_get<NL>(I.pool + (size_t)s * NQ * pstride, pstride, x);
+                el_put<NL>(st, s, x);
+            }
+            left -= trips;
+        }
+        WIDE_FENCE();
+    }
+    return lfl;
+}
+
+// An instance's flag byte: the OR of its lanes' flags, through its word of
+// the wave's LDS flag words.
+__device__ __forceinline__ uint32_t instance_flags(uint32_t *s_lflags, uint32_t g, uint32_t j,
+                                                   uint32_t lfl)
+{
+    if (j == 0)
+        s_lflags[g] = 0;
+    WIDE_FENCE();
+    if (lfl)
+        atomicOr(&s_lflags[g], lfl);
+    WIDE_FENCE();
+    const uint32_t v = s_lflags[g];
+    WIDE_FENCE();
+    return v;
+}
+
 template <int NL>
 __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide Wd,
                                                   int phys_a, int phys_b, int phys_c)
@@ -586,8 +782,6 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
     const uint32_t konst_rows = A.n_consts * NQ;
     RT *s_regs = s_konst + konst_rows;
     const size_t wave_rows = (size_t)Wd.nvregs * ipw * NQ;
-    RT *inst_regs = s_regs + wave * wave_rows + g * NQ;   // register r at + r * ipw * NQ
-    const size_t rstride = (size_t)ipw * NQ;
     uint32_t *s_words = reinterpret_cast<uint32_t *>(s_regs + waves * wave_rows);
     uint32_t *s_lflags = s_words + 2 + wave * kWave;        // per instance
 
@@ -601,10 +795,25 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
             s_words[threadIdx.x] = 0;
         __syncthreads();
     }
-    const uint4 *recs = Wd.recs_in_lds ? s_recs : Wd.recs;
+    WideInst<NL> I;
+    I.recs = Wd.recs_in_lds ? s_recs : Wd.recs;
+    I.nsteps = Wd.nsteps;
+    I.W = W;
+    I.j = j;
+    I.top_step = Wd.top_step;
+    I.nvregs = Wd.nvregs;
+    I.seg = B.seg;
+    I.reach = A.reach;
+    I.n_in = A.n_in;
+    I.n_out = A.n_out;
+    I.konst = s_konst;
+    I.regs = s_regs + wave * wave_rows + g * NQ;
+    I.rstride = (size_t)ipw * NQ;
     const uint64_t islot = (uint64_t)(blockIdx.x * waves + wave) * ipw + g;
-    RT *my_pool = A.pool ? reinterpret_cast<RT *>(A.pool) + islot : nullptr;
-    const size_t pstride = A.npool;
+    I.pool = A.pool ? reinterpret_cast<RT *>(A.pool) + islot : nullptr;
+    I.pstride = A.npool;
+    const uint32_t *const src[3] = {A.a, A.b, A.c};
+    const int phys[3] = {phys_a, phys_b, phys_c};
     uint32_t fl = 0, status = 0;
 
     for (;;) {
@@ -617,172 +826,87 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
         const uint64_t idx = (uint64_t)tile * ipw + g;
         const bool valid = idx < B.n;
         const uint32_t id = valid ? B.order[idx] : 0u;
-        uint32_t left = valid ? B.steps[id] : 0u;
+        const uint32_t left = valid ? B.steps[id] : 0u;
         uint32_t *st = B.state + (size_t)id * A.n_in * NL;
-        uint32_t lfl = 0;
-        while (wave_any(left > 0)) {
-            const bool keep = left > 0;
-            const uint32_t trips = left < B.seg ? left : B.seg;
-            // a segment's start: registers +0, r0..r2 from the streams, the
-            // scratch cleared to reach, the state as the scratch-in block
-            {
-                uint32_t z[NL];
-                zero<NL>(z);
-                for (uint32_t r = j; r < Wd.nvregs; r += W)
-                    row_put<NL>(inst_regs + r * rstride, 1, z);
-                WIDE_FENCE();
-                if (j == 0 && valid) {
-                    const uint32_t *src[3] = {A.a, A.b, A.c};
-                    const int phys[3] = {phys_a, phys_b, phys_c};
-                    for (int k = 0; k < 3; k++)
-                        if (src[k] && phys[k] >= 0) {
-                            uint32_t x[NL];
-                            el_get<NL>(src[k], id, x);
-                            row_put<NL>(inst_regs + (uint32_t)phys[k] * rstride, 1, x);
-                        }
-                }
-                if (keep && my_pool) {
-                    for (uint32_t s = j; s < A.reach; s += W)
-                        row_put<NL>(my_pool + (size_t)s * NQ * pstride, pstride, z);
-                    for (uint32_t s = j; s < A.n_in; s += W) {
-                        uint32_t x[NL];
-                        el_get<NL>(st, s, x);
-                        row_put<NL>(my_pool + (size_t)s * NQ * pstride, pstride, x);
-                    }
-                }
-                WIDE_FENCE();
-            }
-            // the steps
-            bool active = keep;
-            int park = 0;
-            uint32_t left0 = 0, left1 = 0, left2 = 0, left3 = 0;
-            uint32_t body0 = 0, body1 = 0, body2 = 0, body3 = 0;
-            int level = 0;
-            uint32_t pc = 0;
-            while (pc < Wd.nsteps) {
-                pc = uni(pc);
-                const uint4 r0 = recs[(size_t)pc * W];
-                const uint32_t x0 = uni(r0.x);
-                const bool en = active && park == 0;
-                if (WR_KIND(x0) == WK_CTRL) {
-                    const uint32_t op = WR_OP(x0), aux = uni(r0.y), imm = uni(r0.w);
-                    if (op == SEQ_HALT) {
-                        pc = Wd.nsteps;
-                    } else if (op == SEQ_REPEAT) {
-                        const uint32_t t = pc == Wd.top_step ? trips : imm;
-                        const bool enter = en && t != 0;
-                        if (!wave_any(enter)) {
-                            pc = aux + 1;
-                            continue;
-                        }
-                        if (en && !enter)
-                            park = level + 1;
-                        switch (level) {
-                        case 0: left0 = t; body0 = pc + 1; break;
-                        case 1: left1 = t; body1 = pc + 1; break;
-                        case 2: left2 = t; body2 = pc + 1; break;
-                        default: left3 = t; body3 = pc + 1; break;
-                        }
-                        level++;
-                        pc++;
-                    } else {                                  // ENDREP
-                        bool again = false;
-                        if (en) {
-                            uint32_t l;
-                            switch (level) {
-                            case 1: l = --left0; break;
-                            case 2: l = --left1; break;
-                            case 3: l = --left2; break;
-                            default: l = --left3; break;
-                            }
-                            again = l > 0;
-                            if (!again)
-                                park = level;
-                        }
-                        if (wave_any(again)) {
-                            pc = level == 1 ? body0 : level == 2 ? body1 : level == 3 ? body2
-                                                                                   : body3;
-                        } else {
-                            if (park == level)
-                                park = 0;
-                            level--;
-                            pc++;
-                        }
-                    }
-                    continue;
-                }
-                const uint4 rec = j == 0 ? r0 : recs[(size_t)pc * W + j];
-                const uint32_t kind = WR_KIND(rec.x);
-                if (en && kind != WK_NOP) {
-                    if (kind == WK_ALU) {
-                        uint32_t xa[NL], xb[NL], xc[NL], o[NL], f = 0;
-                        const uint32_t ia = WR_A(rec.y), ib = WR_B(rec.y), ic = WR_C(rec.z);
-                        if (WR_KA(rec.x)) row_get<NL>(s_konst + ia * NQ, 1, xa);
-                        else row_get<NL>(inst_regs + ia * rstride, 1, xa);
-                        if (WR_KB(rec.x)) row_get<NL>(s_konst + ib * NQ, 1, xb);
-                        else row_get<NL>(inst_regs + ib * rstride, 1, xb);
-                        if (WR_KC(rec.x)) row_get<NL>(s_konst + ic * NQ, 1, xc);
-                        else row_get<NL>(inst_regs + ic * rstride, 1, xc);
-                        const uint32_t op = WR_OP(rec.x);
-                        if (op <= 3) {
-                            // the lanes of a bundle mix FMA, ADD, SUB and MUL: steer each
-                            // to the FMA it is defined as (host/src/softfloat.c sf_steer:
-                            // ADD a*1+c, SUB a*1+(-c), MUL a*b+(+-0, the product's sign))
-                            // so all of them run one path
-                            if (op == OP_ADD || op == OP_SUB) {
-                                enc_one<NL>(xb);
-                                if (op == OP_SUB)
-                                    xc[NL - 1] ^= 0x80000000u;
-                            } else if (op == OP_MUL) {
-                                const uint32_t sg = (xa[NL - 1] ^ xb[NL - 1]) & 0x80000000u;
-                                zero<NL>(xc);
-                                xc[NL - 1] = sg;
-                            }
-                            fma_core<NL>(OP_FMA, xa, xb, xc, (int)WR_RND(rec.x), o, f);
-                        } else {
-                            misc_op<NL>((int)op, xa, xb, xc, o, f);
-                        }
-                        row_put<NL>(inst_regs + WR_DST(rec.z) * rstride, 1, o);
-                        lfl |= f;
-                    } else if (kind == WK_LOAD) {
-                        uint32_t x[NL];
-                        row_get<NL>(my_pool + (size_t)WR_A(rec.y) * NQ * pstride, pstride, x);
-                        row_put<NL>(inst_regs + WR_DST(rec.z) * rstride, 1, x);
-                    } else if (kind == WK_STORE) {
-                        uint32_t x[NL];
-                        row_get<NL>(inst_regs + WR_A(rec.y) * rstride, 1, x);
-                        row_put<NL>(my_pool + (size_t)WR_DST(rec.z) * NQ * pstride, pstride, x);
-                    } else {                                  // MOVE
-                        uint32_t x[NL];
-                        row_get<NL>(inst_regs + WR_A(rec.y) * rstride, 1, x);
-                        row_put<NL>(inst_regs + WR_DST(rec.z) * rstride, 1, x);
-                    }
-                }
-                WIDE_FENCE();
-                pc++;
-            }
-            // the scratch-out block back into the state (each slot by its owner lane)
-            if (keep) {
-                for (uint32_t s = j; s < A.n_out; s += W) {
-                    uint32_t x[NL];
-                    row_get<NL>(my_pool + (size_t)s * NQ * pstride, pstride, x);
-                    el_put<NL>(st, s, x);
-                }
-                left -= trips;
-            }
-            WIDE_FENCE();
-        }
-        // the instance's flags: its lanes' OR
-        if (j == 0)
-            s_lflags[g] = 0;
-        WIDE_FENCE();
-        if (lfl)
-            atomicOr(&s_lflags[g], lfl);
-        WIDE_FENCE();
+        const uint32_t lfl = wide_run<NL>(I, valid, left, st, src, phys, id);
+        const uint32_t inst = instance_flags(s_lflags, g, j, lfl);
         if (j == 0 && valid && B.lflags)
-            B.lflags[id] = (uint8_t)s_lflags[g];
+            B.lflags[id] = (uint8_t)inst;
         fl |= lfl;
-        WIDE_FENCE();
+    }
+    flush_words(s_words, A.words, fl, status);
+}
+
+// Instances of several wide programs of one format in one launch.  Each wave
+// owns an LDS region of M.region_rows rows: the constants of the program of
+// its current tile, then the register files of the tile's instances.  A
+// wave's scratch slots are instance slots [gwave * 64, gwave * 64 + 64) of
+// the pool, instance g of a tile taking slot g.
+template <int NL>
+__global__ __launch_bounds__(256) void k_seq_multi(SeqArgs A, SeqMulti M)
+{
+    using RT = typename Row<NL>::T;
+    constexpr int NQ = Row<NL>::NQ;
+    extern __shared__ uint4 smem[];
+    const uint32_t waves = blockDim.x / kWave;
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    RT *region = reinterpret_cast<RT *>(smem) + (size_t)wave * M.region_rows;
+    uint32_t *s_words =
+        reinterpret_cast<uint32_t *>(reinterpret_cast<RT *>(smem) + (size_t)waves * M.region_rows);
+    uint32_t *s_lflags = s_words + 2 + wave * kWave;
+    if (threadIdx.x < 2)
+        s_words[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t gwave = (uint64_t)blockIdx.x * waves + wave;
+    const uint32_t *const src[3] = {nullptr, nullptr, nullptr};
+    const int phys[3] = {-1, -1, -1};
+    uint32_t fl = 0, status = 0, staged = 0xFFFFFFFFu;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0)
+            tile = atomicAdd(M.queue, 1u);
+        tile = uni(__shfl(tile, 0));
+        if (tile >= M.ntiles)
+            break;
+        const uint4 T = M.tiles[tile];
+        const uint32_t pi = uni(T.x), first = uni(T.y), count = uni(T.z);
+        const SeqMProg *P = M.progs + pi;
+        const uint32_t log2w = uni(P->log2w), n_consts = uni(P->n_consts);
+        const uint32_t W = 1u << log2w, ipw = kWave >> log2w;
+        const uint32_t g = lane >> log2w, j = lane & (W - 1);
+        if (pi != staged) {
+            const RT *gk = reinterpret_cast<const RT *>(P->bank);
+            for (uint32_t i = lane; i < n_consts * NQ; i += kWave)
+                region[i] = gk[i];
+            WIDE_FENCE();
+            staged = pi;
+        }
+        WideInst<NL> I;
+        I.recs = P->recs;
+        I.nsteps = uni(P->nsteps);
+        I.W = W;
+        I.j = j;
+        I.top_step = uni(P->top_step);
+        I.nvregs = uni(P->nvregs);
+        I.seg = uni(P->seg);
+        I.reach = uni(P->reach);
+        I.n_in = uni(P->n_in);
+        I.n_out = I.n_in;
+        I.konst = region;
+        I.regs = region + (size_t)n_consts * NQ + g * NQ;
+        I.rstride = (size_t)ipw * NQ;
+        I.pool = A.pool ? reinterpret_cast<RT *>(A.pool) + gwave * kWave + g : nullptr;
+        I.pstride = A.npool;
+        const bool valid = g < count;
+        const uint32_t pos = first + (valid ? g : 0u);
+        uint32_t *st = valid ? reinterpret_cast<uint32_t *>(M.saddr[pos]) : nullptr;
+        const uint32_t left = valid ? M.steps[pos] : 0u;
+        const uint32_t lfl = wide_run<NL>(I, valid, left, st, src, phys, pos);
+        const uint32_t inst = instance_flags(s_lflags, g, j, lfl);
+        if (j == 0 && valid && M.lflags)
+            M.lflags[pos] = (uint8_t)inst;
+        fl |= lfl;
     }
     flush_words(s_words, A.words, fl, status);
 }
@@ -1019,6 +1143,37 @@ hipError_t launch_wide(const SeqArgs &A, const SeqBatch &B, const SeqWide &Wd, i
 }
 
 template <int NL>
+hipError_t launch_multi(const SeqArgs &A, const SeqMulti &M, int waves, int blocks, size_t lds,
+                        hipStream_t st)
+{
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_multi<NL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_seq_multi<NL>), dim3((uint32_t)blocks), dim3(waves * kWave), lds, st,
+                       A, M);
+    return hipGetLastError();
+}
+
+template <int NL>
+int multi_occupancy(int waves, size_t lds)
+{
+    int nb = 0;
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_multi<NL>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_seq_multi<NL>, waves * kWave, lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return nb > 0 ? nb : 1;
+}
+
+template <int NL>
 int wide_occupancy(int waves, size_t lds)
 {
     int nb = 0;
@@ -1157,5 +1312,44 @@ extern "C" int cftk_seq_wide_blocks_per_cu(int fmt, uint32_t n_consts, uint32_t 
     case 1: return wide_occupancy<2>(waves, lds);
     case 2: return wide_occupancy<4>(waves, lds);
     default: return wide_occupancy<8>(waves, lds);
+    }
+}
+
+extern "C" size_t cftk_seq_row_bytes(int fmt)
+{
+    switch (fmt) {
+    case 0: return sizeof(Row<1>::T);
+    case 1: return sizeof(Row<2>::T);
+    case 2: return sizeof(Row<4>::T);
+    default: return sizeof(Row<8>::T);
+    }
+}
+
+extern "C" size_t cftk_seq_multi_lds(int fmt, uint32_t region_rows, int waves)
+{
+    return (size_t)waves * region_rows * cftk_seq_row_bytes(fmt) + 8 +
+           (size_t)waves * kWave * 4;
+}
+
+extern "C" hipError_t cftk_seq_multi(int fmt, const SeqArgs *A, const SeqMulti *M, int waves,
+                                     int blocks, hipStream_t st)
+{
+    const size_t lds = cftk_seq_multi_lds(fmt, M->region_rows, waves);
+    switch (fmt) {
+    case 0: return launch_multi<1>(*A, *M, waves, blocks, lds, st);
+    case 1: return launch_multi<2>(*A, *M, waves, blocks, lds, st);
+    case 2: return launch_multi<4>(*A, *M, waves, blocks, lds, st);
+    default: return launch_multi<8>(*A, *M, waves, blocks, lds, st);
+    }
+}
+
+extern "C" int cftk_seq_multi_blocks_per_cu(int fmt, uint32_t region_rows, int waves)
+{
+    const size_t lds = cftk_seq_multi_lds(fmt, region_rows, waves);
+    switch (fmt) {
+    case 0: return multi_occupancy<1>(waves, lds);
+    case 1: return multi_occupancy<2>(waves, lds);
+    case 2: return multi_occupancy<4>(waves, lds);
+    default: return multi_occupancy<8>(waves, lds);
     }
 }

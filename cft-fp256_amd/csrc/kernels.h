@@ -63,6 +63,14 @@ size_t cftk_seq_wide_lds(int fmt, uint32_t n_consts, uint32_t nsteps_in_lds, uin
 int cftk_seq_wide_blocks_per_cu(int fmt, uint32_t n_consts, uint32_t nsteps_in_lds,
                                 uint32_t width, uint32_t nvregs, int waves);
 
+// The multi-program runner; region_rows in rows of the format's LDS row
+// type (16 bytes for binary128/256, 8 for binary64, 4 for binary32).
+hipError_t cftk_seq_multi(int fmt, const SeqArgs *A, const SeqMulti *M, int waves, int blocks,
+                          hipStream_t st);
+size_t cftk_seq_multi_lds(int fmt, uint32_t region_rows, int waves);
+size_t cftk_seq_row_bytes(int fmt);
+int cftk_seq_multi_blocks_per_cu(int fmt, uint32_t region_rows, int waves);
+
 // LDS bytes one interpreter workgroup takes.
 size_t cftk_seq_lds(int fmt, uint32_t n_insns_in_lds, uint32_t n_consts, uint32_t nregs,
                     int waves);

@@ -106,3 +106,32 @@ struct SeqWide {
     uint32_t top_step;                // the segment REPEAT's step
     int recs_in_lds;
 };
+
+// The multi-program runner (k_seq_multi): instances of several wide
+// programs of one format in one launch.  A program's entry (64 bytes):
+struct SeqMProg {
+    const uint4 *recs;                // nsteps x width records
+    const uint32_t *bank;             // its constants, n_consts elements
+    uint32_t nsteps;
+    uint32_t log2w;                   // lanes an instance takes: 1 << log2w
+    uint32_t nvregs;
+    uint32_t top_step;
+    uint32_t n_consts;
+    uint32_t n_in;                    // == n_out: the state's slots
+    uint32_t reach;                   // scratch slots cleared a segment
+    uint32_t seg;                     // trips a segment
+    uint32_t pad[4];
+};
+
+// Tiles are (program, first position, instances) triples handed out longest
+// first; a position names an instance's state and trips.
+struct SeqMulti {
+    const SeqMProg *progs;
+    const uint4 *tiles;               // x program, y first position, z count
+    const uint64_t *saddr;            // per position: its state's device address
+    const uint32_t *steps;            // per position: trips
+    uint8_t *lflags;                  // per position: flag byte, or NULL
+    uint32_t *queue;                  // tile counter, 0 at launch
+    uint32_t ntiles;
+    uint32_t region_rows;             // LDS rows a wave takes (constants + registers)
+};

@@ -44,6 +44,83 @@ struct cft_program {
     int wide_eligible = -1;          // -1 not yet known
 };
 
+// A persistent batched launch's inputs: device pointers (state and lane
+// flags written in place), host steps and lane order (alive until the
+// stream passes the launch).
+struct BatchIO {
+    const void *bank = nullptr, *a = nullptr, *b = nullptr, *c = nullptr;
+    void *state = nullptr;
+    uint8_t *lflags = nullptr;
+    const uint32_t *steps = nullptr;
+    const uint32_t *order = nullptr;
+    size_t n = 0;
+    uint32_t lanes_per_wave = 0, waves_per_group = 0, lanes_per_instance = 0;
+    bool graph = false;
+};
+
+// The persistent batched runner enqueued on ex's stream (cft_batch.cpp).
+cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io,
+                            const char *who, int *width_out);
+
+// The wide schedule of `prog` at `width`, compiled and uploaded on first use
+// (the device current); NULL when the program does not schedule there.
+WideDev *wide_at(cft_program *prog, int width);
+
+// Whether the program runs wide (pure: ALU, static scratch, loops).
+bool prog_wide_eligible(cft_program *prog);
+
+// One program's share of a multi-program launch.
+struct MultiPart {
+    cft_program *prog = nullptr;
+    const void *dbank = nullptr;      // its constants on the device, or NULL
+    std::vector<uint64_t> saddr;      // each lane's state (device address)
+    std::vector<uint32_t> steps;      // each lane's trips
+    uint32_t width = 0;               // lanes an instance takes: 0 for the planner's
+                                      // choice; on return, the width used
+};
+
+// A multi-program launch's host side, alive until the stream passes it:
+// positions are the lanes with trips, each part's longest first.
+struct MultiRun {
+    std::vector<SeqMProg> progs;
+    std::vector<uint32_t> tiles;      // 4 words a tile
+    std::vector<uint64_t> saddr;
+    std::vector<uint32_t> steps;
+    std::vector<uint32_t> part_of, lane_of;
+    std::vector<double> body;         // per part: bundles a trip at its width
+    std::vector<uint8_t> blob;        // the tables, when not staged through pinned memory
+    uint32_t waves = 0, region_rows = 0;
+    uint64_t blocks = 0;
+    double makespan = 0, longest = 0;   // the planner's estimate, in bundles of one wave
+    bool launched = false;
+};
+
+// Pinned host memory through which multi_launch uploads its tables, so the
+// upload does not wait on the stream; kept by the caller until the launch has
+// passed (then reusable).
+struct PinnedStage {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage &) = delete;
+    PinnedStage &operator=(const PinnedStage &) = delete;
+    PinnedStage(PinnedStage &&o) noexcept : p(o.p), cap(o.cap)
+    {
+        o.p = nullptr;
+        o.cap = 0;
+    }
+    ~PinnedStage();
+};
+
+// Plan (widths, LDS region, grid) and enqueue one k_seq_multi launch on
+// ex's stream; dlflags (device, one byte a position: at most the parts'
+// lanes) or NULL.  Every part's program must be wide-eligible and of one
+// format.  The tables go up through `pin` when given, else from pageable
+// memory.  Does not wait.
+cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *dlflags,
+                        uint32_t waves_per_group, MultiRun &run, const char *who,
+                        PinnedStage *pin = nullptr);
+
 // Release a program's captured graph (the device current).
 void seq_drop_graph(cft_program *p);
 

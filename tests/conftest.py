@@ -136,3 +136,12 @@ def device():
     d = Device("hip:0")
     yield d
     d.close()
+
+
+@pytest.fixture(scope="session")
+def deep_device():
+    """A handle at scratch depth 2048 (Lorenz-96 rings past 255 slots)."""
+    from cftamd import Device
+    d = Device("hip:0", scratch_depth=2048)
+    yield d
+    d.close()

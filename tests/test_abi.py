@@ -1,4 +1,4 @@
-"""The C-ABI library loads, exports every entry point include/cft.h declares,
+"""The C-ABI library loads, exports every entry point include/*.h declares,
 audits its opcode/format names, and applies the argument rules that need no
 GPU.  No compute calls: this runs without a GPU."""
 
@@ -11,22 +11,23 @@ import pytest
 
 from conftest import PKG, REPO
 
-HEADER = os.path.join(REPO, "include", "cft.h")
+HEADERS = [os.path.join(REPO, "include", h) for h in ("cft.h", "cft_mi355x.h")]
 LIB = os.path.join(PKG, "lib", "libcft.so")
 
 
 def declared_symbols():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"CFT_API\s+[\w\s\*]*?\b(cft_\w+)\s*\(", text)))
+    text = "".join(open(h).read() for h in HEADERS)
+    return sorted(set(re.findall(r"CFT_API\s+[\w\s\*]*?\b(cftx?_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
     assert os.path.exists(LIB), "libcft.so is not built"
     out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
                          text=True, check=True).stdout
-    exported = set(re.findall(r"\bT (cft_\w+)", out))
+    exported = set(re.findall(r"\bT (cftx?_\w+)", out))
     syms = declared_symbols()
-    assert len(syms) >= 30
+    assert len(syms) >= 50
+    assert {"cftx_program_run_multi", "cftx_server_open", "cftx_program_run_batch"} <= set(syms)
     missing = [s for s in syms if s not in exported]
     assert not missing, missing
 

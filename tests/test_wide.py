@@ -18,19 +18,16 @@ from seqlib import ESZ, rand_values, run_oracle
 WK_ALU, WK_LOAD, WK_STORE, WK_MOVE, WK_NOP, WK_CTRL = range(6)
 
 
-def wide(image, width, depth=256):
+def wide(image, width, depth=256, max_vregs=0):
     L = _lib.lib()
-    L.cftx_program_wide.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32,
-                                    ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t,
-                                    ctypes.POINTER(ctypes.c_uint32)]
     im = np.frombuffer(bytes(image), dtype=np.uint8).copy()
     info = (ctypes.c_uint32 * 6)()
-    st = L.cftx_program_wide(im.ctypes.data, im.nbytes, depth, width, None, 0, info)
+    st = L.cftx_program_wide(im.ctypes.data, im.nbytes, depth, width, max_vregs, None, 0, info)
     if st != 0:
         return st, None, None
     recs = np.zeros(info[0] * width * 4, dtype=np.uint32)
-    st = L.cftx_program_wide(im.ctypes.data, im.nbytes, depth, width, recs.ctypes.data,
-                             info[0] * width, info)
+    st = L.cftx_program_wide(im.ctypes.data, im.nbytes, depth, width, max_vregs,
+                             recs.ctypes.data, info[0] * width, info)
     return st, recs.reshape(info[0], width, 4), list(info)
 
 
