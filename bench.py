@@ -16,7 +16,12 @@ Beside it, in the same JSON line:
               ensemble beside the persistent form on that sample (bit-equal),
               and the reference's CPU program runs on a bounded sample;
   lorenz96    Lorenz-96 RK4 in binary64 and binary256, 8,192 instances, ring
-              sizes 8..256, on a 1/100 sample of the steps;
+              sizes 8..256, steps uniform 1e3..2e4: every ring size in one
+              multi-program launch, and on a 1/100 sample of the steps that
+              launch beside one batched launch a ring size (bit-equal);
+  serve       the serving loop over a Poisson stream at 2,000 requests/s:
+              p50/p99 latency, throughput, every job checked against the job
+              run on its own, the plain one-job-a-launch form on a prefix;
   cpu_baseline  the reference's CPU FMA on this host's cores, same inputs,
               compared bit for bit.
 
@@ -63,6 +68,7 @@ def parse():
     ap.add_argument("--skip-ops", action="store_true", help="FMA only")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--skip-rk4", action="store_true", help="no Lorenz-63/96 sections")
+    ap.add_argument("--skip-serve", action="store_true", help="no serving section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES, three times")
     return ap.parse_args()
@@ -297,37 +303,107 @@ def rk4_section(dev, lib, args, threads):
         b.free()
     prog.free()
 
-    # Lorenz-96: 8,192 instances, ring 8..256, a 1/1000 sample of the steps
+    # Lorenz-96: 8,192 instances, ring 8..256, every ring size in one launch;
+    # the planner's knobs tuned on a 1/100 sample of the steps first
+    from cftamd import last_plan, run_multi, tune_multi
     with Device("hip:0", scratch_depth=2048) as d2:
         for fmt, name in ((1, "lorenz96_fp64"), (3, "lorenz96_fp256")):
             groups = odes.lorenz96(8192, seed=args.seed, fmt=fmt)
-            tot = comp = 0
-            flags = 0
-            progs = []
-            for k, gimg, gbank, ids, gstate, gsteps in groups:
-                # load, schedule and warm each ring size outside the timed region
-                p = Program(d2, gimg)
-                p.run_batch(gstate.copy(), np.ones_like(gsteps), bank=gbank)
-                progs.append(p)
-            widths = set()
+            progs = [Program(d2, g[1]) for g in groups]
+            for p in progs:                 # schedules built outside the timed region
+                p.prepare(0x7F)
+            tot = sum(int(g[5].sum()) for g in groups)
+            comp = sum(int(g[5].sum()) * g[0] for g in groups)
+            samp = [np.maximum(1, g[5] // 100).astype(np.uint32) for g in groups]
+            knobs, trials = tune_multi(d2, [dict(prog=p, bank=g[2], state=g[4], steps=s2)
+                                            for p, g, s2 in zip(progs, groups, samp)])
+            parts = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=g[5])
+                     for p, g in zip(progs, groups)]
             t0 = time.perf_counter()
-            for p, (k, gimg, gbank, ids, gstate, gsteps) in zip(progs, groups):
-                gs = np.maximum(1, gsteps // 100).astype(np.uint32)
-                f, _s = p.run_batch(gstate, gs, bank=gbank)
-                widths.add(int(lib.cftx_batch_width(d2.handle)))
-                flags |= f
-                tot += int(gs.sum())
-                comp += int(gs.sum()) * k
+            fl, _s, widths = run_multi(d2, parts, **knobs)
             secs = time.perf_counter() - t0
+            plan = last_plan(d2)
+            # on the sample: the same one launch against one batched launch a
+            # ring size, bit for bit
+            sp = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=s2)
+                  for p, g, s2 in zip(progs, groups, samp)]
+            t0 = time.perf_counter()
+            run_multi(d2, sp, **knobs)
+            t_multi = time.perf_counter() - t0
+            eq = True
+            t0 = time.perf_counter()
+            outs = []
+            for p, g, s2 in zip(progs, groups, samp):
+                st2 = g[4].copy()
+                p.run_batch(st2, s2, bank=g[2])
+                outs.append(st2)
+            t_perk = time.perf_counter() - t0
+            for o, q in zip(outs, sp):
+                eq &= bool(np.array_equal(o, q["state"]))
+            stot = int(sum(int(x.sum()) for x in samp))
             for p in progs:
                 p.free()
-            out[name] = {"instances": 8192, "sample": "steps // 100 (10..200 a lane), "
-                         "one persistent launch a ring size, 32 ring sizes",
-                         "total_steps": tot, "seconds": round(secs, 4),
-                         "steps_per_s": round(tot / secs, 1),
-                         "component_steps_per_s": round(comp / secs, 1), "flags": flags,
-                         "lanes_per_instance": sorted(widths)}
-            log(f"bench: {name} {tot / secs:.0f} steps/s ({secs:.2f} s)")
+            out[name] = {"instances": 8192, "ring_sizes": len(groups), "total_steps": tot,
+                         "seconds": round(secs, 4), "steps_per_s": round(tot / secs, 1),
+                         "component_steps_per_s": round(comp / secs, 1), "flags": fl,
+                         "launches": 1, "lanes_per_instance_by_ring": {
+                             str(g[0]): w for g, w in zip(groups, widths)},
+                         "plan": plan, "tuned_knobs": knobs,
+                         "tuning_trials_s": [[t, k] for t, k in trials],
+                         "sample": {"steps": "steps // 100 (at least 1)", "total_steps": stot,
+                                    "one_launch_s": round(t_multi, 4),
+                                    "launch_a_ring_s": round(t_perk, 4),
+                                    "speedup": round(t_perk / t_multi, 2), "bit_equal": eq}}
+            log(f"bench: {name} {tot / secs / 1e6:.2f} M steps/s ({secs:.2f} s, one launch); "
+                f"sample one launch {t_multi:.3f} s vs a launch a ring {t_perk:.3f} s")
+    return out
+
+
+def serve_section(args):
+    """The serving loop over a Poisson stream at 2,000 requests/s for one
+    second (cftamd/serve.py states the mix and sizes): p50/p99 latency and
+    throughput, every job checked against the same job run on its own, and
+    the plain form - one worker, one job a launch, program jobs run whole -
+    on the stream's first 20 ms beside the batched loop on the same
+    requests."""
+    from cftamd import Device
+    from cftamd import serve as S
+    out = {}
+    with Device("hip:0", scratch_depth=2048) as dev:
+        reqs = S.requests(seed=args.seed, rate=2000, duration=1.0)
+        pools = S.Pools(dev, reqs, seed=args.seed)
+        srv = S.Server(dev, workers=4)
+        res, wall = S.serve(srv, pools, reqs)
+        stats = srv.stats()
+        srv.close()
+        summ = S.summarize(reqs, res, wall)
+        log(f"bench: serve p50 {summ['p50_ms']} ms p99 {summ['p99_ms']} ms over "
+            f"{len(reqs)} requests ({wall:.2f} s)")
+        v = S.verify(dev, pools, reqs, res, alone_every=100)
+        log(f"bench: serve verification {v}")
+        pre = [r for r in reqs if r["t"] < 0.02]
+        pools.states.host[: pools.state0.nbytes] = pools.state0
+        pools.states.to_device()
+        srv = S.Server(dev, workers=1)
+        srv.set_batching(1)
+        res_p, wall_p = S.serve(srv, pools, pre)
+        srv.close()
+        plain = S.summarize(pre, res_p, wall_p)
+        same = S.summarize(pre, res[: len(pre)], wall)
+        log(f"bench: serve plain form p50 {plain['p50_ms']} ms p99 {plain['p99_ms']} ms on "
+            f"{len(pre)} requests; batched on the same: p50 {same['p50_ms']} p99 "
+            f"{same['p99_ms']}")
+        pools.free()
+        out = {"workload": "Poisson 2,000 requests/s for 1 s, seed %d" % args.seed,
+               "requests": summ["requests"], "p50_ms": summ["p50_ms"],
+               "p99_ms": summ["p99_ms"], "wall_s": summ["wall_s"],
+               "served_req_s": summ["throughput_req_s"], "by_kind": summ["by_kind"],
+               "elements": summ["elements"], "ode_steps": summ["ode_steps"],
+               "statuses_ok": summ["statuses_ok"], "server": stats, "verification": v,
+               "plain_form_prefix": {"requests": len(pre), "p50_ms": plain["p50_ms"],
+                                     "p99_ms": plain["p99_ms"], "wall_s": plain["wall_s"],
+                                     "batched_p50_ms": same["p50_ms"],
+                                     "batched_p99_ms": same["p99_ms"]}}
     return out
 
 
@@ -524,6 +600,10 @@ def main():
     if not args.skip_rk4:
         rk4 = rk4_section(dev, lib, args, args.cpu_threads or min(16, os.cpu_count() or 1))
 
+    serve = None
+    if not args.skip_serve:
+        serve = serve_section(args)
+
     cpu = None
     bit_exact = None
     if not args.skip_cpu:
@@ -563,6 +643,7 @@ def main():
         "bit_exact_vs_cpu": bit_exact,
         "ops": ops,
         "rk4": rk4,
+        "serve": serve,
     }
     for bf in bufs:
         bf.free()

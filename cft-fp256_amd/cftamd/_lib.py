@@ -133,6 +133,11 @@ class CftxBatchArgs(ctypes.Structure):
 BATCH_PERSISTENT, BATCH_PER_OP = 0, 1
 
 
+class CftxMultiOpts(ctypes.Structure):
+    _fields_ = [("struct_size", c_size_t), ("waves_per_group", c_uint32),
+                ("waves_per_cu", c_uint32), ("alone", ctypes.c_double)]
+
+
 class CftxBatchPart(ctypes.Structure):
     _fields_ = [("struct_size", c_size_t), ("prog", c_void_p),
                 ("bank", c_void_p), ("bank_bytes", c_size_t),
@@ -218,8 +223,9 @@ def _bind(lib):
         "cft_program_digest": ([V, V, c_size_t, V], c_int),
         "cftx_program_check": ([V, c_size_t, c_uint32, POINTER(CftProgramInfo)], c_int),
         "cftx_program_run_batch": ([V, POINTER(CftxBatchArgs)], c_int),
-        "cftx_program_run_multi": ([V, POINTER(CftxBatchPart), c_size_t, c_uint32, P, P], c_int),
+        "cftx_program_run_multi": ([V, POINTER(CftxBatchPart), c_size_t, V, P, P], c_int),
         "cftx_program_prepare": ([V, c_uint32, P], c_int),
+        "cftx_multi_last_plan": ([V, POINTER(ctypes.c_uint64)], c_int),
         "cftx_batch_launches": ([V], ctypes.c_uint64),
         "cftx_batch_width": ([V], c_uint32),
         "cftx_server_open": ([V, c_int, c_size_t, POINTER(c_void_p)], c_int),

@@ -13,7 +13,7 @@ from ctypes import byref, c_uint32, c_void_p
 import numpy as np
 
 from . import _lib
-from ._lib import check, lib
+from ._lib import CftError, check, lib
 
 
 def _ptr(x):
@@ -224,10 +224,11 @@ class Program:
             pass
 
 
-def run_multi(dev, parts, waves_per_group=0):
+def run_multi(dev, parts, waves_per_group=0, waves_per_cu=0, alone=0.0):
     """cftx_program_run_multi over parts, each a dict with prog (a Program),
     state (updated in place), steps, and optionally bank, lane_flags and
-    lanes_per_instance.  Returns (flags, status, widths used)."""
+    lanes_per_instance; the planner's knobs after.  Returns (flags, status,
+    widths used)."""
     arr = (_lib.CftxBatchPart * len(parts))()
     keep = []
     nb = lambda x: 0 if x is None else (x.nbytes if isinstance(x, np.ndarray) else x.bytes)
@@ -244,9 +245,44 @@ def run_multi(dev, parts, waves_per_group=0):
         P.lane_flags = _ptr(p.get("lane_flags"))
         P.lanes_per_instance = p.get("lanes_per_instance", 0)
     fl, bus = c_uint32(), c_uint32()
-    check(lib().cftx_program_run_multi(dev.handle, arr, len(parts), waves_per_group,
+    opts = _lib.CftxMultiOpts(ctypes.sizeof(_lib.CftxMultiOpts), waves_per_group, waves_per_cu,
+                              alone)
+    check(lib().cftx_program_run_multi(dev.handle, arr, len(parts), byref(opts),
                                        byref(fl), byref(bus)), "cftx_program_run_multi")
     return fl.value, bus.value, [arr[k].width_used for k in range(len(parts))]
+
+
+def last_plan(dev):
+    """cftx_multi_last_plan: the handle's last multi-program launch plan."""
+    out = (ctypes.c_uint64 * 6)()
+    check(lib().cftx_multi_last_plan(dev.handle, out), "cftx_multi_last_plan")
+    keys = ("waves_per_group", "groups", "lds_rows_per_wave", "tiles", "groups_per_cu",
+            "planned_bundles")
+    return dict(zip(keys, [int(v) for v in out]))
+
+
+def tune_multi(dev, parts, trials=None):
+    """The autotuner: each planner setting in `trials` (dicts of run_multi's
+    knobs; by default lone-wave fractions 0.3..1.0 with the planner's own
+    waves a CU, and 4 and 8 waves a CU) runs on copies of the parts' states,
+    and the fastest is returned with every trial's seconds.  Run it on a
+    sample of the workload (steps scaled down): the planner's choices depend
+    on the ratios of the steps, not their size."""
+    import time
+    if trials is None:
+        trials = [dict(alone=a) for a in (0.3, 0.5, 0.7, 0.9, 1.0)]
+        trials += [dict(alone=a, waves_per_cu=k) for a in (0.5, 0.9) for k in (4, 8)]
+    results = []
+    for t in trials:
+        trial = [dict(p, state=np.array(p["state"], copy=True)) for p in parts]
+        try:
+            t0 = time.perf_counter()
+            run_multi(dev, trial, **t)
+            results.append((time.perf_counter() - t0, t))
+        except CftError:
+            continue
+    results.sort(key=lambda r: r[0])
+    return results[0][1], [(round(s, 5), t) for s, t in results]
 
 
 def check_image(image, scratch_depth=0):

@@ -34,30 +34,57 @@ void seq_drop_graph(cft_program *p)
     memset(p->gkey, 0, sizeof p->gkey);
 }
 
-WideDev *wide_at(cft_program *prog, int width)
+uint32_t wide_default_budget(const cft_program *prog, int width)
 {
-    auto it = prog->wide.find(width);
-    if (it != prog->wide.end())
-        return it->second.failed ? nullptr : &it->second;
-    WideDev &wd = prog->wide[width];
     const size_t ipw = 64 / width;
-    const uint32_t budget =
-        (uint32_t)std::min<size_t>(65535, (prog->dev->lds_max / 2) / (ipw * prog->esz));
-    if (!seq_wide_compile(prog, width, budget, &wd.wp)) {
-        wd.failed = true;
-        return nullptr;
+    return (uint32_t)std::min<size_t>(65535, (prog->dev->lds_max / 2) / (ipw * prog->esz));
+}
+
+WideDev *wide_at(cft_program *prog, int width, uint32_t max_vregs)
+{
+    const uint32_t budget = max_vregs ? std::min(max_vregs, wide_default_budget(prog, width))
+                                      : wide_default_budget(prog, width);
+    // the widest window within budget; each (width, window) schedule is built
+    // once and shared by every budget that picks it.  The narrowest window
+    // (the fewest registers) is built first: a budget it exceeds is refused
+    // without building the others.
+    auto at = [&](int window) -> WideDev & {
+        window = std::max(window, width);
+        const uint64_t key = (uint64_t)width << 32 | (uint32_t)window;
+        auto it = prog->wide.find(key);
+        if (it == prog->wide.end()) {
+            it = prog->wide.emplace(key, WideDev()).first;
+            if (!seq_wide_compile_at(prog, width, window, &it->second.wp))
+                it->second.failed = true;
+        }
+        return it->second;
+    };
+    {
+        const WideDev &narrow = at(kWideWindows[kWideWindowCount - 1]);
+        if (narrow.failed || narrow.wp.nvregs > budget)
+            return nullptr;
     }
-    const size_t bytes = wd.wp.recs.size() * sizeof(WideRec);
-    if (hipMalloc(&wd.d_recs, bytes) != hipSuccess ||
-        hipMemcpy(wd.d_recs, wd.wp.recs.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipGetLastError();
-        if (wd.d_recs)
-            (void)hipFree(wd.d_recs);
-        wd.d_recs = nullptr;
-        wd.failed = true;
-        return nullptr;
+    for (int i = 0; i < kWideWindowCount; i++) {
+        WideDev &wd = at(kWideWindows[i]);
+        if (wd.failed)
+            return nullptr;
+        if (wd.wp.nvregs > budget)
+            continue;
+        if (!wd.d_recs) {
+            const size_t bytes = wd.wp.recs.size() * sizeof(WideRec);
+            if (hipMalloc(&wd.d_recs, bytes) != hipSuccess ||
+                hipMemcpy(wd.d_recs, wd.wp.recs.data(), bytes, hipMemcpyHostToDevice) !=
+                    hipSuccess) {
+                (void)hipGetLastError();
+                if (wd.d_recs)
+                    (void)hipFree(wd.d_recs);
+                wd.d_recs = nullptr;
+                return nullptr;
+            }
+        }
+        return &wd;
     }
-    return &wd;
+    return nullptr;
 }
 
 bool prog_wide_eligible(cft_program *prog)
@@ -286,6 +313,19 @@ cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io
                                               recs_lds ? wp.nsteps : 0, width, wp.nvregs,
                                               waves) > ex->lds_max)
             waves /= 2;
+        if (!io.waves_per_group) {
+            // the group size that keeps the most waves resident on a CU
+            int best = 0;
+            for (int w = waves; w >= 1; w /= 2) {
+                const int r = w * cftk_seq_wide_blocks_per_cu(prog->fmt, prog->n_consts,
+                                                              recs_lds ? wp.nsteps : 0, width,
+                                                              wp.nvregs, w);
+                if (r > best) {
+                    best = r;
+                    waves = w;
+                }
+            }
+        }
         if (cftk_seq_wide_lds(prog->fmt, prog->n_consts, recs_lds ? wp.nsteps : 0, width,
                               wp.nvregs, waves) > ex->lds_max) {
             cft_set_error("%s: %lu registers an instance over %d lanes do not fit one "

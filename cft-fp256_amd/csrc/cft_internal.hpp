@@ -74,6 +74,9 @@ struct cft_device {
     uint64_t timed_launches = 0;
     uint64_t batch_launches = 0;   // kernel launches the last batched run made
     uint32_t batch_width = 0;      // lanes an instance took in it
+    // the last multi-program launch's plan: waves a group, groups, LDS rows a
+    // wave, tiles, groups a CU holds, planned makespan (bundles)
+    uint64_t multi_plan[6] = {0, 0, 0, 0, 0, 0};
 };
 
 // Bracket a compute-kernel launch with the handle's timer events (no-ops

@@ -13,14 +13,21 @@
 // lanes sorted longest first and cut into tiles of 64 / W, a tile costs
 // (its longest lane's trips) x (bundles a trip at W) in bundles of one wave.
 // The launch's makespan is taken as
-//     max(sum of tile costs / wave slots issuing at once, longest tile)
-// where the slots are the CUs x min(4, waves a CU holds): one wave a SIMD
-// issues as fast as the SIMD does.  Waves a CU holds follow from the LDS
-// region a wave needs, the largest over the programs of
-//     (constants + nvregs(W) x 64 / W) rows.
-// The planner tries each distinct longest-tile bound L: every program takes
-// its cheapest width whose longest tile is within L (the fastest one if none
-// is), and the bound with the smallest makespan wins.
+//     max(sum of tile costs / issue slots, longest tile / a)
+// where a wave alone on a SIMD issues at a fraction a (0.9) of the SIMD's
+// rate and two or more keep it busy, so a CU gives sum over its 4 SIMDs of
+// min(1, a x waves on it).  Waves a CU holds follow from the LDS region a
+// wave needs, the largest over the programs of
+//     (constants + nvregs(W) x 64 / W) rows,
+// and a program's candidates at each W include schedules built within
+// register budgets that let 4, 6 or 8 waves share a CU's LDS.
+// The planner tries each cap on the region (the waves a CU must hold, 1 to
+// 12) and each distinct longest-tile bound L: every program takes its
+// cheapest candidate within the cap whose longest tile is within L (the
+// fastest within the cap if none is), and the plan with the smallest
+// makespan wins.  The caller may fix the group size, the waves a CU holds
+// and the lone-wave fraction a: cftx_program_run_multi's options, which an
+// autotuner sweeps on a sample.
 #include "seq_prog.hpp"
 
 #include <string.h>
@@ -49,6 +56,44 @@ size_t wave_lds(int fmt, uint32_t rows)
     return (size_t)rows * cftk_seq_row_bytes(fmt) + 4 * 64 + 8;
 }
 
+// A wave alone on its SIMD issues at about this fraction of the SIMD's rate,
+// a format: a binary256 bundle is long straight-line integer code, so one
+// wave nearly fills the SIMD (measured: one Lorenz-63 wave alone takes 2.55
+// us a bundle, a full ensemble 2.86 us a bundle a SIMD); the short bundles of
+// the narrow formats leave LDS and record-load latency exposed.
+constexpr double kAlone[4] = {0.4, 0.5, 0.75, 0.9};
+
+// SIMDs' worth of full-rate issue the device gives when every wave needs
+// `rows` rows of LDS: waves a CU holds (LDS, and 12 for registers), spread
+// over its 4 SIMDs, each SIMD at min(1, alone x its waves).
+double issue_slots(const cft_device *ex, int fmt, uint32_t rows, double lds_cu, double alone)
+{
+    const int per_cu = (int)std::min(12.0, std::floor(lds_cu / (double)wave_lds(fmt, rows)));
+    double cu = 0;
+    for (int s = 0; s < 4; s++) {
+        const int k = per_cu / 4 + (s < per_cu % 4 ? 1 : 0);
+        cu += std::min(1.0, alone * k);
+    }
+    return (double)ex->num_cus * std::max(cu, alone);
+}
+
+// Register budgets a program is scheduled within at width w: the default,
+// and tighter ones that let 4, 6 or 8 waves share a CU's LDS (fewer
+// registers, usually a few more bundles).
+std::vector<uint32_t> budgets(const cft_program *p, int w)
+{
+    std::vector<uint32_t> out{0};
+    const uint32_t ipw = 64 / (uint32_t)w, nq = rows_per_el(p->fmt);
+    for (int per_cu : {4, 6, 8}) {
+        const double rows = ((double)p->dev->lds_max / per_cu - 4 * 64 - 8) /
+                            (double)cftk_seq_row_bytes(p->fmt);
+        const double v = (rows / nq - p->n_consts) / ipw;
+        if (v >= 1 && v < wide_default_budget(p, w))
+            out.push_back((uint32_t)v);
+    }
+    return out;
+}
+
 } // namespace
 
 PinnedStage::~PinnedStage()
@@ -58,9 +103,10 @@ PinnedStage::~PinnedStage()
 }
 
 cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *dlflags,
-                        uint32_t waves_per_group, MultiRun &run, const char *who,
+                        const MultiTune &tune, MultiRun &run, const char *who,
                         PinnedStage *pin)
 {
+    const uint32_t waves_per_group = tune.waves_per_group;
     if (parts.empty())
         return CFT_OK;
     const int fmt = parts[0].prog->fmt;
@@ -80,6 +126,15 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
                          [&](uint32_t x, uint32_t y) { return P.steps[x] > P.steps[y]; });
     }
 
+    run = MultiRun();
+    {
+        bool any = false;
+        for (const std::vector<uint32_t> &L : lanes)
+            any |= !L.empty();
+        if (!any)
+            return CFT_OK;             // no lane has trips: nothing to launch
+    }
+
     // candidates a part
     std::vector<std::vector<Cand>> cands(np);
     for (size_t p = 0; p < np; p++) {
@@ -87,26 +142,35 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
         const std::vector<uint32_t> &L = lanes[p];
         if (L.empty())
             continue;
-        for (uint32_t w = 1; w <= 64; w *= 2) {
+        uint32_t fewest = 0xFFFFFFFFu;          // the fewest registers a schedule took
+        for (uint32_t w = 64; w >= 1; w /= 2) {
             if (P.width && P.width != w)
                 continue;
-            WideDev *wd = wide_at(P.prog, (int)w);
-            if (!wd)
-                continue;
-            Cand c;
-            c.width = w;
-            c.wd = wd;
             const uint32_t ipw = 64 / w;
+            if (fewest != 0xFFFFFFFFu &&
+                wave_lds(fmt, (P.prog->n_consts + fewest * ipw) * nq) > ex->lds_max)
+                continue;                      // narrower widths hold more instances a wave
             double trips = 0;
             for (size_t k = 0; k < L.size(); k += ipw)
                 trips += P.steps[L[k]];
-            const double body = std::max<uint32_t>(1, wd->wp.body_steps);
-            c.work = trips * body;
-            c.longest = (double)P.steps[L[0]] * body;
-            c.rows = (P.prog->n_consts + wd->wp.nvregs * ipw) * nq;
-            if (wave_lds(fmt, c.rows) > ex->lds_max)
-                continue;
-            cands[p].push_back(c);
+            std::vector<uint32_t> seen;
+            for (uint32_t b : budgets(P.prog, (int)w)) {
+                WideDev *wd = wide_at(P.prog, (int)w, b);
+                if (!wd || std::find(seen.begin(), seen.end(), wd->wp.nvregs) != seen.end())
+                    continue;
+                seen.push_back(wd->wp.nvregs);
+                fewest = std::min(fewest, wd->wp.nvregs);
+                Cand c;
+                c.width = w;
+                c.wd = wd;
+                const double body = std::max<uint32_t>(1, wd->wp.body_steps);
+                c.work = trips * body;
+                c.longest = (double)P.steps[L[0]] * body;
+                c.rows = (P.prog->n_consts + wd->wp.nvregs * ipw) * nq;
+                if (wave_lds(fmt, c.rows) > ex->lds_max)
+                    continue;
+                cands[p].push_back(c);
+            }
         }
         if (cands[p].empty()) {
             if (P.width)
@@ -119,7 +183,10 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
         }
     }
 
-    // the bound search
+    // the search: for each cap on the LDS region (the waves a CU must hold)
+    // and each longest-tile bound, every program takes its cheapest candidate
+    // within both (the fastest within the cap if none is within the bound)
+    const double alone = tune.alone > 0 ? std::min(1.0, tune.alone) : kAlone[fmt];
     std::vector<double> bounds;
     for (auto &cs : cands)
         for (const Cand &c : cs)
@@ -127,41 +194,62 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
     std::sort(bounds.begin(), bounds.end());
     bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
     const double lds_cu = (double)ex->lds_max;
+    std::vector<uint32_t> caps;
+    for (int k = 1; k <= 12; k++) {
+        if (tune.min_waves_per_cu && k != (int)tune.min_waves_per_cu)
+            continue;
+        const double r = (lds_cu / k - 4 * 64 - 8) / (double)cftk_seq_row_bytes(fmt);
+        if (r >= 1)
+            caps.push_back((uint32_t)r);
+    }
     std::vector<int> best_pick(np, -1);
     double best = -1, best_long = 0;
     uint32_t best_rows = 0;
-    for (double bound : bounds) {
-        std::vector<int> pick(np, -1);
-        double work = 0, longest = 0;
-        uint32_t rows = 0;
-        for (size_t p = 0; p < np; p++) {
-            const std::vector<Cand> &cs = cands[p];
-            if (cs.empty())
-                continue;
-            int k = -1;
-            for (int i = 0; i < (int)cs.size(); i++)
-                if (cs[i].longest <= bound &&
-                    (k < 0 || cs[i].work < cs[k].work ||
-                     (cs[i].work == cs[k].work && cs[i].rows < cs[k].rows)))
-                    k = i;
-            if (k < 0)
+    for (uint32_t cap : caps) {
+        for (double bound : bounds) {
+            std::vector<int> pick(np, -1);
+            double work = 0, longest = 0;
+            uint32_t rows = 0;
+            bool fits = true;
+            for (size_t p = 0; p < np && fits; p++) {
+                const std::vector<Cand> &cs = cands[p];
+                if (cs.empty())
+                    continue;
+                int k = -1;
                 for (int i = 0; i < (int)cs.size(); i++)
-                    if (k < 0 || cs[i].longest < cs[k].longest)
+                    if (cs[i].rows <= cap && cs[i].longest <= bound &&
+                        (k < 0 || cs[i].work < cs[k].work ||
+                         (cs[i].work == cs[k].work && cs[i].rows < cs[k].rows)))
                         k = i;
-            pick[p] = k;
-            work += cs[k].work;
-            longest = std::max(longest, cs[k].longest);
-            rows = std::max(rows, cs[k].rows);
+                if (k < 0)
+                    for (int i = 0; i < (int)cs.size(); i++)
+                        if (cs[i].rows <= cap && (k < 0 || cs[i].longest < cs[k].longest))
+                            k = i;
+                if (k < 0) {
+                    fits = false;
+                    break;
+                }
+                pick[p] = k;
+                work += cs[k].work;
+                longest = std::max(longest, cs[k].longest);
+                rows = std::max(rows, cs[k].rows);
+            }
+            if (!fits)
+                break;                  // a tighter bound does not loosen the cap
+            const double span =
+                std::max(work / issue_slots(ex, fmt, rows, lds_cu, alone), longest / alone);
+            if (best < 0 || span < best * 0.999 || (span <= best * 1.001 && rows < best_rows)) {
+                best = span;
+                best_long = longest;
+                best_rows = rows;
+                best_pick = pick;
+            }
         }
-        const double per_cu = std::min(8.0, std::floor(lds_cu / (double)wave_lds(fmt, rows)));
-        const double slots = (double)ex->num_cus * std::max(1.0, std::min(4.0, per_cu));
-        const double span = std::max(work / slots, longest);
-        if (best < 0 || span < best * 0.999 || (span <= best * 1.001 && rows < best_rows)) {
-            best = span;
-            best_long = longest;
-            best_rows = rows;
-            best_pick = pick;
-        }
+    }
+    if (best < 0) {
+        cft_set_error("%s: no plan holds %lu waves a CU (min_waves_per_cu)", who,
+                      (unsigned long)tune.min_waves_per_cu);
+        return CFT_ERR_UNSUPPORTED;
     }
 
     // positions, tiles and program entries
@@ -243,9 +331,26 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
     const uint64_t ntiles = tl.size();
     while (waves > 1 && (uint64_t)waves > ntiles)
         waves /= 2;
+    if (!waves_per_group) {
+        // the group size that keeps the most waves resident on a CU
+        int best = 0;
+        for (int w = waves; w >= 1; w /= 2) {
+            const int r = w * cftk_seq_multi_blocks_per_cu(fmt, run.region_rows, w);
+            if (r > best) {
+                best = r;
+                waves = w;
+            }
+        }
+    }
     const int per_cu = cftk_seq_multi_blocks_per_cu(fmt, run.region_rows, waves);
     run.waves = (uint32_t)waves;
     run.blocks = std::min<uint64_t>((ntiles + waves - 1) / waves, (uint64_t)ex->num_cus * per_cu);
+    ex->multi_plan[0] = (uint64_t)waves;
+    ex->multi_plan[1] = run.blocks;
+    ex->multi_plan[2] = run.region_rows;
+    ex->multi_plan[3] = ntiles;
+    ex->multi_plan[4] = (uint64_t)per_cu;
+    ex->multi_plan[5] = (uint64_t)run.makespan;
     const uint64_t npool = run.blocks * (uint64_t)waves * 64;
 
     cft_status st = CFT_OK;
@@ -360,7 +465,7 @@ cft_status check_part(cft_device *dev, const cftx_batch_part &P, size_t k, int f
 } // namespace
 
 extern "C" CFT_API cft_status cftx_program_run_multi(cft_device *dev, cftx_batch_part *parts,
-                                                     size_t nparts, uint32_t waves_per_group,
+                                                     size_t nparts, const cftx_multi_opts *opts,
                                                      uint32_t *flags_out, uint32_t *bus_out)
 {
     cft_clear_error();
@@ -370,11 +475,23 @@ extern "C" CFT_API cft_status cftx_program_run_multi(cft_device *dev, cftx_batch
         cft_set_error("%s: %s", kWho, !dev ? "dev is NULL" : "no parts");
         return CFT_ERR_INVALID_ARGUMENT;
     }
-    if (waves_per_group != 0 && waves_per_group != 1 && waves_per_group != 2 &&
-        waves_per_group != 4) {
-        cft_set_error("%s: waves_per_group %lu is not 0, 1, 2 or 4", kWho,
-                      (unsigned long)waves_per_group);
-        return CFT_ERR_INVALID_ARGUMENT;
+    MultiTune tune;
+    if (opts) {
+        if (opts->struct_size != sizeof(cftx_multi_opts)) {
+            cft_set_error("%s was given a %zu-byte cftx_multi_opts and this library's is %zu",
+                          kWho, opts->struct_size, sizeof(cftx_multi_opts));
+            return CFT_ERR_INVALID_ARGUMENT;
+        }
+        if ((opts->waves_per_group != 0 && opts->waves_per_group != 1 &&
+             opts->waves_per_group != 2 && opts->waves_per_group != 4) ||
+            opts->waves_per_cu > 12 || !(opts->alone >= 0 && opts->alone <= 1)) {
+            cft_set_error("%s: options out of range (waves_per_group 0, 1, 2 or 4; "
+                          "waves_per_cu 0..12; alone 0..1)", kWho);
+            return CFT_ERR_INVALID_ARGUMENT;
+        }
+        tune.waves_per_group = opts->waves_per_group;
+        tune.min_waves_per_cu = opts->waves_per_cu;
+        tune.alone = opts->alone;
     }
     if (!parts[0].prog || parts[0].struct_size != sizeof(cftx_batch_part)) {
         cft_set_error("%s: part 0 has no program or a foreign struct_size", kWho);
@@ -484,7 +601,7 @@ extern "C" CFT_API cft_status cftx_program_run_multi(cft_device *dev, cftx_batch
     }
     uint8_t *dlf = (uint8_t *)dev->stage[ROLE_LFLAGS].p;
     MultiRun run;
-    if ((st = multi_launch(dev, mp, dlf, waves_per_group, run, kWho)) != CFT_OK)
+    if ((st = multi_launch(dev, mp, dlf, tune, run, kWho)) != CFT_OK)
         return st;
     std::vector<uint8_t> hlf(run.saddr.size());
     for (size_t k = 0; k < nparts && e == hipSuccess; k++)
@@ -550,12 +667,33 @@ extern "C" CFT_API cft_status cftx_program_prepare(cft_program *prog, uint32_t w
                       "flag control); it runs one lane an instance", who);
         return CFT_ERR_UNSUPPORTED;
     }
-    uint32_t ok = 0;
-    for (int k = 0; k <= 6; k++)
-        if ((widths >> k) & 1u)
-            if (wide_at(prog, 1 << k))
+    // widest first; a width whose wave cannot hold its instances' registers
+    // in LDS even at the fewest registers seen is not built (as the planner
+    // skips it)
+    uint32_t ok = 0, fewest = 0xFFFFFFFFu;
+    const uint32_t nq = rows_per_el(prog->fmt);
+    for (int k = 6; k >= 0; k--) {
+        if (!((widths >> k) & 1u))
+            continue;
+        const uint32_t ipw = 64u >> k;
+        if (fewest != 0xFFFFFFFFu &&
+            wave_lds(prog->fmt, (prog->n_consts + fewest * ipw) * nq) > prog->dev->lds_max)
+            continue;
+        for (uint32_t b : budgets(prog, 1 << k))
+            if (WideDev *wd = wide_at(prog, 1 << k, b)) {
                 ok |= 1u << k;
+                fewest = std::min(fewest, wd->wp.nvregs);
+            }
+    }
     if (ready)
         *ready = ok;
+    return CFT_OK;
+}
+
+extern "C" CFT_API cft_status cftx_multi_last_plan(cft_device *dev, uint64_t out[6])
+{
+    if (!dev || !out)
+        return CFT_ERR_INVALID_ARGUMENT;
+    memcpy(out, dev->multi_plan, sizeof dev->multi_plan);
     return CFT_OK;
 }

@@ -516,8 +516,8 @@ cft_status launch_slice(cftx_server *S, Worker &w, bool *launched)
         if (e != hipSuccess)
             break;
         R.lf0 = off;
-        if ((st = multi_launch(ex, R.parts, dlf + off, 0, R.run, kWho, &sl.pin[R.fmt])) !=
-            CFT_OK)
+        if ((st = multi_launch(ex, R.parts, dlf + off, MultiTune(), R.run, kWho,
+                               &sl.pin[R.fmt])) != CFT_OK)
             return st;
         off += R.run.saddr.size();
     }

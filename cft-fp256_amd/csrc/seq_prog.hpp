@@ -39,8 +39,8 @@ struct cft_program {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     uint64_t gkey[24] = {};
-    // wide schedules by width, built on first use
-    std::map<int, WideDev> wide;
+    // wide schedules by (width << 32 | scheduling window), built on first use
+    std::map<uint64_t, WideDev> wide;
     int wide_eligible = -1;          // -1 not yet known
 };
 
@@ -62,9 +62,14 @@ struct BatchIO {
 cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io,
                             const char *who, int *width_out);
 
-// The wide schedule of `prog` at `width`, compiled and uploaded on first use
-// (the device current); NULL when the program does not schedule there.
-WideDev *wide_at(cft_program *prog, int width);
+// The wide schedule of `prog` at `width` within `max_vregs` registers an
+// instance (0: the most half the device's LDS holds for a wave's instances),
+// compiled and uploaded on first use (the device current); NULL when the
+// program does not schedule there.
+WideDev *wide_at(cft_program *prog, int width, uint32_t max_vregs = 0);
+
+// The register budget wide_at uses for max_vregs 0.
+uint32_t wide_default_budget(const cft_program *prog, int width);
 
 // Whether the program runs wide (pure: ALU, static scratch, loops).
 bool prog_wide_eligible(cft_program *prog);
@@ -112,13 +117,20 @@ struct PinnedStage {
     ~PinnedStage();
 };
 
+// The planner's knobs (0: its own choice).
+struct MultiTune {
+    uint32_t waves_per_group = 0;     // 1, 2 or 4
+    uint32_t min_waves_per_cu = 0;    // plan for exactly this many waves a CU's LDS holds
+    double alone = 0;                 // the issue fraction of a wave alone on a SIMD
+};
+
 // Plan (widths, LDS region, grid) and enqueue one k_seq_multi launch on
 // ex's stream; dlflags (device, one byte a position: at most the parts'
 // lanes) or NULL.  Every part's program must be wide-eligible and of one
 // format.  The tables go up through `pin` when given, else from pageable
 // memory.  Does not wait.
 cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *dlflags,
-                        uint32_t waves_per_group, MultiRun &run, const char *who,
+                        const MultiTune &tune, MultiRun &run, const char *who,
                         PinnedStage *pin = nullptr);
 
 // Release a program's captured graph (the device current).

@@ -420,7 +420,6 @@ struct Emitter {
     const cft_program *p;
     int W;
     int window;
-    uint32_t max_vregs;
     std::map<uint32_t, VarSet> block_out;
     WideProgram *out;
     uint32_t temps = 0;
@@ -533,31 +532,40 @@ bool seq_wide_eligible(const cft_program *p)
     return true;
 }
 
-bool seq_wide_compile(const cft_program *p, int width, uint32_t max_vregs, WideProgram *out)
+const int kWideWindows[] = {1 << 20, 4096, 2048, 1536, 1024, 768, 512, 384, 256,
+                           128, 64, 32, 16};
+const int kWideWindowCount = (int)(sizeof kWideWindows / sizeof kWideWindows[0]);
+
+bool seq_wide_compile_at(const cft_program *p, int width, int window, WideProgram *out)
 {
     std::vector<Item> items;
     parse(p, 0, program_end(p), items);
-    static const int windows[] = {1 << 20, 4096, 1024, 256, 64, 16};
-    for (int window : windows) {
-        if (window < width)
-            window = width;
+    WideProgram wp;
+    wp.width = width;
+    wp.nregs = p->nregs;
+    Emitter E;
+    E.p = p;
+    E.W = width;
+    E.window = std::max(window, width);
+    E.out = &wp;
+    VarSet exit_live;
+    for (uint32_t s = 0; s < p->n_out && s < (uint32_t)kMaxSlots; s++)
+        exit_live.set(var_slot(s));
+    live_in_seq(p, items, exit_live, E.block_out);
+    E.emit_seq(items, nullptr);
+    if (!E.ok)
+        return false;
+    wp.nvregs = p->nregs + E.temps;
+    *out = std::move(wp);
+    return true;
+}
+
+bool seq_wide_compile(const cft_program *p, int width, uint32_t max_vregs, WideProgram *out)
+{
+    for (int i = 0; i < kWideWindowCount; i++) {
         WideProgram wp;
-        wp.width = width;
-        wp.nregs = p->nregs;
-        Emitter E;
-        E.p = p;
-        E.W = width;
-        E.window = window;
-        E.max_vregs = max_vregs;
-        E.out = &wp;
-        VarSet exit_live;
-        for (uint32_t s = 0; s < p->n_out && s < (uint32_t)kMaxSlots; s++)
-            exit_live.set(var_slot(s));
-        live_in_seq(p, items, exit_live, E.block_out);
-        E.emit_seq(items, nullptr);
-        if (!E.ok)
+        if (!seq_wide_compile_at(p, width, kWideWindows[i], &wp))
             return false;
-        wp.nvregs = p->nregs + E.temps;
         if (wp.nvregs <= max_vregs) {
             *out = std::move(wp);
             return true;

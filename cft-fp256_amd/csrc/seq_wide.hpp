@@ -72,6 +72,15 @@ struct cft_program;
 // Whether the program can run wide (pure: see above).
 bool seq_wide_eligible(const cft_program *p);
 
-// Schedule for `width` lanes an instance; false when a block needs more than
-// `max_vregs` registers even at the narrowest window.
+// The scheduler's lookahead windows, widest first: a narrower window keeps
+// fewer values live (fewer registers) at the cost of some parallelism.
+extern const int kWideWindows[];
+extern const int kWideWindowCount;
+
+// Schedule for `width` lanes an instance at one window; false when the
+// program does not schedule.
+bool seq_wide_compile_at(const cft_program *p, int width, int window, WideProgram *out);
+
+// Schedule for `width` lanes an instance at the widest window within
+// `max_vregs` registers; false when none is.
 bool seq_wide_compile(const cft_program *p, int width, uint32_t max_vregs, WideProgram *out);

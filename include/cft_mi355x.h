@@ -112,11 +112,27 @@ typedef struct cftx_batch_part {
     uint32_t        width_used;          /* out: lanes an instance took */
 } cftx_batch_part;
 
-/* flags_out: the OR of every lane's flags; bus_out: their STATUS bits.
- * waves_per_group 0 (the planner's), 1, 2 or 4. */
+/* The planner's knobs, for an autotuner that sweeps them on a sample (0
+ * everywhere, or a NULL pointer, leaves every choice to the planner):
+ * waves_per_group 1, 2 or 4; waves_per_cu 1..12 plans for exactly that many
+ * waves sharing a CU's LDS; alone in (0, 1] is the share of a SIMD's issue
+ * rate the planner takes a wave alone on it to get. */
+typedef struct cftx_multi_opts {
+    size_t   struct_size;
+    uint32_t waves_per_group;
+    uint32_t waves_per_cu;
+    double   alone;
+} cftx_multi_opts;
+
+/* flags_out: the OR of every lane's flags; bus_out: their STATUS bits. */
 CFT_API cft_status cftx_program_run_multi(cft_device *dev, cftx_batch_part *parts,
-                                          size_t nparts, uint32_t waves_per_group,
+                                          size_t nparts, const cftx_multi_opts *opts,
                                           uint32_t *flags_out, uint32_t *bus_out);
+
+/* The plan of the handle's last multi-program launch: out[0..5] = {waves a
+ * workgroup, workgroups, LDS rows a wave, tiles, workgroups a CU holds, the
+ * planner's makespan estimate in bundles}. */
+CFT_API cft_status cftx_multi_last_plan(cft_device *dev, uint64_t out[6]);
 
 /* Build a pure program's wide schedules ahead of its first batched run: bit
  * k of `widths` asks for 2^k lanes an instance (k <= 6); *ready receives the

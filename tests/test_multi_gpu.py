@@ -6,7 +6,7 @@ an instance; the refusals."""
 import numpy as np
 import pytest
 
-from cftamd import CftError, Program, _lib, odes, run_multi
+from cftamd import CftError, Program, _lib, odes, run_multi, tune_multi
 from cftamd import programs as P
 from seqlib import ESZ, oracle_batch, rand_values
 from test_wide import pure_program
@@ -158,4 +158,40 @@ def test_refusals(device):
                                              steps=np.zeros(4, np.uint32))])
     assert np.array_equal(st, state3) and (fl, status) == (0, 0)
     for q in (p1, p3, impure):
+        q.free()
+
+
+def test_planner_options_change_no_bit(deep_device):
+    device = deep_device
+    groups = odes.lorenz96(64, seed=11, fmt=3, steps_lo=1, steps_hi=40, ks=[8, 48, 136, 256])
+    progs = [Program(device, g[1]) for g in groups]
+    img, bank, state, steps = odes.lorenz63(100, seed=12, fmt=3, steps_lo=1, steps_hi=250)
+    progs.append(Program(device, img))
+
+    def parts():
+        out = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=g[5])
+               for p, g in zip(progs, groups)]
+        out.append(dict(prog=progs[-1], bank=bank, state=state.copy(), steps=steps))
+        return out
+
+    base = parts()
+    f0, s0, _w = run_multi(device, base)
+    best, trials = tune_multi(device, parts())
+    assert trials and best in [t for _s, t in trials]
+    for knobs in (dict(waves_per_group=1), dict(waves_per_group=2, alone=0.3),
+                  dict(waves_per_cu=8), dict(waves_per_cu=2, alone=1.0), best):
+        try:
+            got = parts()
+            f, s, _w = run_multi(device, got, **knobs)
+        except CftError as e:            # a plan no program fits is refused, not run
+            assert e.status == _lib.ERR_UNSUPPORTED, knobs
+            continue
+        assert (f, s) == (f0, s0)
+        for a, b in zip(base, got):
+            assert np.array_equal(a["state"], b["state"]), knobs
+    with pytest.raises(CftError):
+        run_multi(device, parts(), waves_per_cu=13)
+    with pytest.raises(CftError):
+        run_multi(device, parts(), alone=1.5)
+    for q in progs:
         q.free()
