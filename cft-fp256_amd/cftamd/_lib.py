@@ -169,7 +169,7 @@ class CftxServerStats(ctypes.Structure):
                 ("batches", ctypes.c_uint64), ("elem_batches", ctypes.c_uint64),
                 ("program_batches", ctypes.c_uint64), ("max_batch_jobs", ctypes.c_uint64),
                 ("peak_bytes", ctypes.c_uint64), ("slices", ctypes.c_uint64),
-                ("max_slice_lanes", ctypes.c_uint64)]
+                ("max_slice_lanes", ctypes.c_uint64), ("slice_host_s", ctypes.c_double)]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 IN_TREE = os.path.join(os.path.dirname(HERE), "lib", "libcft.so")

@@ -14,6 +14,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include <algorithm>
 #include <new>
 
@@ -148,6 +150,32 @@ cft_status dev_fail(cft_device *dev, hipError_t e, const char *who, const char *
                   who, what, dev->hip_index, hipGetErrorString(e), hipGetErrorName(e));
     (void)hipGetLastError();
     return CFT_ERR_INTERNAL;
+}
+
+cft_status arena_grow(cft_device *dev, DevArena &a, size_t bytes, const char *who)
+{
+    if (bytes <= a.cap)
+        return CFT_OK;
+    size_t want = std::max<size_t>({bytes, a.cap * 2, (size_t)1 << 20});
+    want = (want + 4095) & ~(size_t)4095;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        want = bytes;
+        e = hipMalloc(&p, want);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        cft_set_error("%s: %zu bytes of device memory could not be allocated on GPU %d", who,
+                      bytes, dev->hip_index);
+        return CFT_ERR_OUT_OF_MEMORY;
+    }
+    if (a.p)
+        dev->retired.push_back(a.p);
+    a.p = p;
+    a.cap = want;
+    return CFT_OK;
 }
 
 cft_status arena_reserve(cft_device *dev, DevArena &a, size_t bytes, const char *who)
@@ -352,7 +380,8 @@ extern "C" CFT_API cft_status cftx_kernel_timer_read(cft_device *dev, double *to
 
 // ---- open / close -----------------------------------------------------------
 
-static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft_device **out)
+static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft_device **out,
+                            bool own_queue = false)
 {
     static const char grammar[] = "the artifact is \"hip:\" followed by the device's number in "
                                   "decimal and nothing else - hip:0 is the first GPU";
@@ -419,8 +448,21 @@ static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft
     if (prop.sharedMemPerBlock > 0)
         D->lds_max = prop.sharedMemPerBlock;
     // default priority, always: work on streams of different priorities can
-    // stall each other indefinitely on this platform
-    if ((e = hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking)) != hipSuccess) {
+    // stall each other indefinitely on this platform.  A stream with a CU
+    // mask (here every CU) gets a hardware queue of its own rather than one
+    // shared with other streams of the process, so its kernels never wait
+    // behind another stream's long launch.
+    e = hipErrorUnknown;
+    if (own_queue) {
+        std::vector<uint32_t> mask((D->num_cus + 31) / 32, 0u);
+        for (int c = 0; c < D->num_cus; c++)
+            mask[c / 32] |= 1u << (c % 32);
+        e = hipExtStreamCreateWithCUMask(&D->stream, (uint32_t)mask.size(), mask.data());
+        if (e != hipSuccess)
+            (void)hipGetLastError();
+    }
+    if (e != hipSuccess &&
+        (e = hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking)) != hipSuccess) {
         cft_set_error("cft_open: %s: creating a stream failed: %s", artifact,
                       hipGetErrorString(e));
         (void)hipGetLastError();
@@ -441,11 +483,11 @@ static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft
     return CFT_OK;
 }
 
-cft_status device_open(int hip_index, uint32_t depth, cft_device **out)
+cft_status device_open(int hip_index, uint32_t depth, cft_device **out, bool own_queue)
 {
     char name[32];
     snprintf(name, sizeof name, "hip:%d", hip_index);
-    return open_impl(name, 0, depth, out);
+    return open_impl(name, 0, depth, out, own_queue);
 }
 
 extern "C" CFT_API void *cftx_buffer_device_ptr(cft_device *dev, const void *p, size_t bytes)
@@ -510,6 +552,8 @@ extern "C" CFT_API void cft_close(cft_device *dev)
     }
     for (DevArena &a : dev->stage)
         if (a.p) (void)hipFree(a.p);
+    for (void *p : dev->retired)
+        (void)hipFree(p);
     if (dev->dwords) (void)hipFree(dev->dwords);
     if (dev->hwords) (void)hipHostFree(dev->hwords);
     if (dev->ev0) (void)hipEventDestroy(dev->ev0);

@@ -74,6 +74,9 @@ struct cft_device {
     uint64_t timed_launches = 0;
     uint64_t batch_launches = 0;   // kernel launches the last batched run made
     uint32_t batch_width = 0;      // lanes an instance took in it
+    // arenas replaced while launches that read them were in flight, freed
+    // with the handle (freeing device memory waits for the whole device)
+    std::vector<void *> retired;
     // the last multi-program launch's plan: waves a group, groups, LDS rows a
     // wave, tiles, groups a CU holds, planned makespan (bundles)
     uint64_t multi_plan[6] = {0, 0, 0, 0, 0, 0};
@@ -105,6 +108,11 @@ cft_status dev_fail(cft_device *dev, hipError_t e, const char *who, const char *
 // Grow arena to at least bytes.  CFT_ERR_OUT_OF_MEMORY before anything ran.
 cft_status arena_reserve(cft_device *dev, DevArena &a, size_t bytes, const char *who);
 
+// The same without waiting: a larger arena replaces the old one, which is
+// kept (retired) until the handle closes, so launches still queued on the
+// stream keep reading valid memory.  Grows at least twofold.
+cft_status arena_grow(cft_device *dev, DevArena &a, size_t bytes, const char *who);
+
 // Operand binding.  An input window [p, p+bytes) that lies inside a resident
 // buffer is served from its device copy (refreshed first if stale); any other
 // window is copied into the role's staging arena.  *dptr receives the device
@@ -124,8 +132,9 @@ cft_buffer *find_buffer(cft_device *dev, const void *p, size_t bytes);
 cft_status host_in(cft_device *dev, const void *p, const char *who);
 
 // Another handle on GPU hip_index at scratch depth `depth` (its own stream
-// and staging arenas), for the serving runtime's workers.
-cft_status device_open(int hip_index, uint32_t depth, cft_device **out);
+// and staging arenas), for the serving runtime's workers; own_queue asks for
+// a stream on a hardware queue no other stream shares.
+cft_status device_open(int hip_index, uint32_t depth, cft_device **out, bool own_queue = false);
 
 // Kernel grid for n lanes of one element each.
 int grid_for(const cft_device *dev, size_t n);

@@ -100,6 +100,8 @@ PinnedStage::~PinnedStage()
 {
     if (p)
         (void)hipHostFree(p);
+    for (uint8_t *q : retired)
+        (void)hipHostFree(q);
 }
 
 cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *dlflags,
@@ -202,40 +204,63 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
         if (r >= 1)
             caps.push_back((uint32_t)r);
     }
+    // each part's candidates within a cap, by longest tile, with the running
+    // cheapest: the cheapest within a bound is one binary search away
+    struct Ranked {
+        std::vector<double> longest;    // ascending
+        std::vector<int> idx;           // candidate, same order
+        std::vector<int> cheapest;      // cheapest among [0, i]
+    };
     std::vector<int> best_pick(np, -1);
     double best = -1, best_long = 0;
     uint32_t best_rows = 0;
     for (uint32_t cap : caps) {
+        std::vector<Ranked> rk(np);
+        bool fits = true;
+        for (size_t p = 0; p < np && fits; p++) {
+            const std::vector<Cand> &cs = cands[p];
+            if (cs.empty())
+                continue;
+            Ranked &R = rk[p];
+            for (int i = 0; i < (int)cs.size(); i++)
+                if (cs[i].rows <= cap)
+                    R.idx.push_back(i);
+            if (R.idx.empty()) {
+                fits = false;
+                break;
+            }
+            std::sort(R.idx.begin(), R.idx.end(),
+                      [&](int a, int b) { return cs[a].longest < cs[b].longest; });
+            int c = -1;
+            for (int i : R.idx) {
+                R.longest.push_back(cs[i].longest);
+                if (c < 0 || cs[i].work < cs[c].work ||
+                    (cs[i].work == cs[c].work && cs[i].rows < cs[c].rows))
+                    c = i;
+                R.cheapest.push_back(c);
+            }
+        }
+        if (!fits)
+            continue;
         for (double bound : bounds) {
-            std::vector<int> pick(np, -1);
             double work = 0, longest = 0;
             uint32_t rows = 0;
-            bool fits = true;
-            for (size_t p = 0; p < np && fits; p++) {
-                const std::vector<Cand> &cs = cands[p];
-                if (cs.empty())
+            std::vector<int> pick(np, -1);
+            for (size_t p = 0; p < np; p++) {
+                const Ranked &R = rk[p];
+                if (R.idx.empty())
                     continue;
-                int k = -1;
-                for (int i = 0; i < (int)cs.size(); i++)
-                    if (cs[i].rows <= cap && cs[i].longest <= bound &&
-                        (k < 0 || cs[i].work < cs[k].work ||
-                         (cs[i].work == cs[k].work && cs[i].rows < cs[k].rows)))
-                        k = i;
-                if (k < 0)
-                    for (int i = 0; i < (int)cs.size(); i++)
-                        if (cs[i].rows <= cap && (k < 0 || cs[i].longest < cs[k].longest))
-                            k = i;
-                if (k < 0) {
-                    fits = false;
-                    break;
-                }
+                const size_t n_in =
+                    std::upper_bound(R.longest.begin(), R.longest.end(), bound) -
+                    R.longest.begin();
+                // the cheapest within the bound, else the fastest within the cap
+                const int k = n_in ? R.cheapest[n_in - 1] : R.idx[0];
+                const Cand &c = cands[p][k];
                 pick[p] = k;
-                work += cs[k].work;
-                longest = std::max(longest, cs[k].longest);
-                rows = std::max(rows, cs[k].rows);
+                work += c.work;
+                longest = std::max(longest, c.longest);
+                rows = std::max(rows, c.rows);
             }
-            if (!fits)
-                break;                  // a tighter bound does not loosen the cap
             const double span =
                 std::max(work / issue_slots(ex, fmt, rows, lds_cu, alone), longest / alone);
             if (best < 0 || span < best * 0.999 || (span <= best * 1.001 && rows < best_rows)) {
@@ -342,7 +367,9 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
             }
         }
     }
-    const int per_cu = cftk_seq_multi_blocks_per_cu(fmt, run.region_rows, waves);
+    int per_cu = cftk_seq_multi_blocks_per_cu(fmt, run.region_rows, waves);
+    if (tune.max_waves_per_cu)
+        per_cu = std::max(1, std::min(per_cu, (int)tune.max_waves_per_cu / waves));
     run.waves = (uint32_t)waves;
     run.blocks = std::min<uint64_t>((ntiles + waves - 1) / waves, (uint64_t)ex->num_cus * per_cu);
     ex->multi_plan[0] = (uint64_t)waves;
@@ -354,8 +381,10 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
     const uint64_t npool = run.blocks * (uint64_t)waves * 64;
 
     cft_status st = CFT_OK;
-    if (reach_max &&
-        (st = arena_reserve(ex, ex->stage[ROLE_POOL], npool * reach_max * esz, who)) != CFT_OK)
+    auto reserve = [&](DevArena &a, size_t bytes) {
+        return pin ? arena_grow(ex, a, bytes, who) : arena_reserve(ex, a, bytes, who);
+    };
+    if (reach_max && (st = reserve(ex->stage[ROLE_POOL], npool * reach_max * esz)) != CFT_OK)
         return st;
     // the four tables as one block: programs | tiles | state addresses | trips
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -366,17 +395,20 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
     uint8_t *host = nullptr;
     if (pin) {
         if (pin->cap < total) {
-            if (pin->p)
-                (void)hipHostFree(pin->p);
-            pin->p = nullptr;
-            pin->cap = 0;
-            if (hipHostMalloc((void **)&pin->p, total) != hipSuccess) {
+            // grow twofold; the old buffer is kept, not freed (a free would
+            // wait for the device)
+            const size_t want = std::max<size_t>({total, pin->cap * 2, (size_t)1 << 16});
+            uint8_t *q = nullptr;
+            if (hipHostMalloc((void **)&q, want) != hipSuccess) {
                 (void)hipGetLastError();
                 cft_set_error("%s: %zu bytes of pinned host memory could not be allocated",
-                              who, total);
+                              who, want);
                 return CFT_ERR_OUT_OF_MEMORY;
             }
-            pin->cap = total;
+            if (pin->p)
+                pin->retired.push_back(pin->p);
+            pin->p = q;
+            pin->cap = want;
         }
         host = pin->p;
     } else {
@@ -387,7 +419,7 @@ cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *
     memcpy(host + o_tiles, run.tiles.data(), b_tiles);
     memcpy(host + o_saddr, run.saddr.data(), b_saddr);
     memcpy(host + o_steps, run.steps.data(), b_steps);
-    if ((st = arena_reserve(ex, ex->stage[ROLE_MPROGS], total, who)) != CFT_OK)
+    if ((st = reserve(ex->stage[ROLE_MPROGS], total)) != CFT_OK)
         return st;
     uint8_t *dev_tables = (uint8_t *)ex->stage[ROLE_MPROGS].p;
     hipError_t ce = hipMemcpyAsync(dev_tables, host, total, hipMemcpyHostToDevice, ex->stream);

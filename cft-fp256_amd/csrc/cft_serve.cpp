@@ -120,6 +120,7 @@ struct Worker {
     uint32_t *hflags = nullptr;          // pinned
     uint8_t *hlf = nullptr;              // pinned lane flags of a whole batch
     size_t hlf_cap = 0;
+    std::vector<uint8_t *> retired_host; // outgrown pinned buffers, freed at close
     double us_per_bundle[4] = {0, 0, 0, 0};   // measured, a format
     std::unordered_map<const cft_program *, double> body;   // bundles a trip, last slice
 };
@@ -154,6 +155,8 @@ struct cftx_server {
     std::map<std::vector<uint8_t>, void *> banks;   // device copies of submitted banks
     // a program's jobs gather on one program worker while it holds any
     std::unordered_map<const cft_program *, int> home;
+    // slices leave room on every CU for the elementwise workers' kernels
+    MultiTune slice_tune;
 };
 
 namespace {
@@ -252,20 +255,22 @@ hipError_t launch_elem(Worker &w)
     return e;
 }
 
-cft_status pinned_bytes(uint8_t *&p, size_t &cap, size_t n)
+// Pinned host bytes, grown at least twofold; an outgrown buffer is kept in
+// `retired` (freeing pinned memory waits for the device) until close.
+cft_status pinned_bytes(uint8_t *&p, size_t &cap, size_t n, std::vector<uint8_t *> &retired)
 {
     if (cap >= n)
         return CFT_OK;
-    if (p)
-        (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-    const size_t want = std::max<size_t>(n, 4096);
-    if (hipHostMalloc((void **)&p, want) != hipSuccess) {
+    const size_t want = std::max<size_t>({n, cap * 2, (size_t)1 << 16});
+    uint8_t *q = nullptr;
+    if (hipHostMalloc((void **)&q, want) != hipSuccess) {
         (void)hipGetLastError();
         cft_set_error("%s: %zu bytes of pinned host memory could not be allocated", kWho, want);
         return CFT_ERR_OUT_OF_MEMORY;
     }
+    if (p)
+        retired.push_back(p);
+    p = q;
     cap = want;
     return CFT_OK;
 }
@@ -290,11 +295,11 @@ cft_status launch_whole(Worker &w)
         B.order[i] = (uint32_t)i;
     std::stable_sort(B.order.begin(), B.order.end(),
                      [&](uint32_t x, uint32_t y) { return B.steps[x] > B.steps[y]; });
-    cft_status st = arena_reserve(ex, ex->stage[ROLE_SIN], n * slots * esz, kWho);
+    cft_status st = arena_grow(ex, ex->stage[ROLE_SIN], n * slots * esz, kWho);
     if (st == CFT_OK)
-        st = arena_reserve(ex, ex->stage[ROLE_LFLAGS], n, kWho);
+        st = arena_grow(ex, ex->stage[ROLE_LFLAGS], n, kWho);
     if (st == CFT_OK)
-        st = pinned_bytes(w.hlf, w.hlf_cap, n);
+        st = pinned_bytes(w.hlf, w.hlf_cap, n, w.retired_host);
     if (st != CFT_OK)
         return st;
     B.dstate = ex->stage[ROLE_SIN].p;
@@ -504,9 +509,9 @@ cft_status launch_slice(cftx_server *S, Worker &w, bool *launched)
     }
     if (lanes == 0)
         return CFT_OK;
-    cft_status st = arena_reserve(ex, ex->stage[ROLE_LFLAGS], lanes, kWho);
+    cft_status st = arena_grow(ex, ex->stage[ROLE_LFLAGS], lanes, kWho);
     if (st == CFT_OK)
-        st = pinned_bytes(sl.hlf, sl.hlf_cap, lanes);
+        st = pinned_bytes(sl.hlf, sl.hlf_cap, lanes, w.retired_host);
     if (st != CFT_OK)
         return st;
     uint8_t *dlf = (uint8_t *)ex->stage[ROLE_LFLAGS].p;
@@ -516,7 +521,7 @@ cft_status launch_slice(cftx_server *S, Worker &w, bool *launched)
         if (e != hipSuccess)
             break;
         R.lf0 = off;
-        if ((st = multi_launch(ex, R.parts, dlf + off, MultiTune(), R.run, kWho,
+        if ((st = multi_launch(ex, R.parts, dlf + off, S->slice_tune, R.run, kWho,
                                &sl.pin[R.fmt])) != CFT_OK)
             return st;
         off += R.run.saddr.size();
@@ -735,7 +740,12 @@ void start_program(cftx_server *S, Worker &w)
     if (w.set.empty())
         return;
     bool launched = false;
+    const double t0 = now_s();
     const cft_status st = launch_slice(S, w, &launched);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stats.slice_host_s += now_s() - t0;
+    }
     if (st != CFT_OK)
         drop_set(S, w, st);
 }
@@ -877,7 +887,9 @@ extern "C" CFT_API cft_status cftx_server_open(cft_device *dev, int workers, siz
         w.index = k;
         w.can = workers == 1 ? (kCanElem | kCanProg) : (k < n_elem ? kCanElem : kCanProg);
         w.first_prog = workers == 1 || k == n_elem;
-        st = device_open(dev->hip_index, dev->scratch_depth, &w.dev);
+        // elementwise workers on hardware queues of their own: their short
+        // launches never queue behind a program slice
+        st = device_open(dev->hip_index, dev->scratch_depth, &w.dev, w.can == kCanElem);
         hipError_t e = hipSuccess;
         if (st == CFT_OK) e = hipEventCreateWithFlags(&w.ev, hipEventDisableTiming);
         for (Slot &sl : w.slots) {
@@ -898,6 +910,12 @@ extern "C" CFT_API cft_status cftx_server_open(cft_device *dev, int workers, siz
             return st;
         }
     }
+    // a CU holds about 12 sequencer waves (registers); the program workers'
+    // slices share 8 of them and leave the rest to elementwise launches,
+    // which would otherwise wait for a slice to end before they start
+    const int n_prog = workers - n_elem;
+    if (n_elem > 0)
+        S->slice_tune.max_waves_per_cu = (uint32_t)std::max(1, 8 / n_prog);
     S->sched = std::thread(scheduler, S);
     *out = S;
     return CFT_OK;
@@ -1114,6 +1132,8 @@ extern "C" CFT_API void cftx_server_close(cftx_server *S)
         if (w.dflags) (void)hipFree(w.dflags);
         if (w.hflags) (void)hipHostFree(w.hflags);
         if (w.hlf) (void)hipHostFree(w.hlf);
+        for (uint8_t *q : w.retired_host)
+            (void)hipHostFree(q);
         if (w.dev) cft_close(w.dev);
     }
     for (auto &kv : S->banks)

@@ -38,15 +38,20 @@ __global__ __launch_bounds__(256) void k_divsqrt(const uint32_t *__restrict__ a,
         for (int k = 0; k < NL; k++)
             d[i * NL + k] = o[k];
     }
-    __shared__ uint32_t s_fl;
-    if (threadIdx.x == 0)
-        s_fl = 0;
-    __syncthreads();
-    if (fl)
-        atomicOr(&s_fl, fl);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_fl)
-        atomicOr(flags_out, s_fl);
+    // the wavefront's OR of its lanes' flags, one atomic for bits not yet set
+    // (no LDS: see flush_flags in k_elem.hip)
+    uint32_t w = 0;
+#pragma unroll
+    for (int b = 0; b < 5; b++)
+        if (__ballot((fl >> b) & 1u))
+            w |= 1u << b;
+    const uint64_t live = __ballot(1);
+    if (w && __lane_id() == (unsigned)__builtin_ctzll(live)) {
+        const uint32_t have = __hip_atomic_load(flags_out, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        if (w & ~have)
+            atomicOr(flags_out, w);
+    }
 }
 
 template <int NL>

@@ -53,19 +53,25 @@ __device__ __forceinline__ void store_el(uint32_t *__restrict__ p, size_t i,
     }
 }
 
-// One workgroup's flags into the call's word: wave OR by LDS atomics, then a
-// single global atomic.
+// One wavefront's flags into the call's word: the OR over its lanes by one
+// ballot a flag bit, then one global atomic from its first active lane, and
+// only for bits the word does not hold yet (after the first waves set them,
+// most waves only read it).  No LDS, so these kernels can start on a CU whose
+// LDS a persistent sequencer launch of another stream holds.
 __device__ __forceinline__ void flush_flags(uint32_t fl, uint32_t *flags_out)
 {
-    __shared__ uint32_t s_fl;
-    if (threadIdx.x == 0)
-        s_fl = 0;
-    __syncthreads();
-    if (fl)
-        atomicOr(&s_fl, fl);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_fl)
-        atomicOr(flags_out, s_fl);
+    uint32_t w = 0;
+#pragma unroll
+    for (int b = 0; b < 5; b++)
+        if (__ballot((fl >> b) & 1u))
+            w |= 1u << b;
+    const uint64_t live = __ballot(1);
+    if (w && __lane_id() == (unsigned)__builtin_ctzll(live)) {
+        const uint32_t have = __hip_atomic_load(flags_out, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        if (w & ~have)
+            atomicOr(flags_out, w);
+    }
 }
 
 // Arithmetic group.  OP is FMA/ADD/SUB/MUL; operands NULL when not read.

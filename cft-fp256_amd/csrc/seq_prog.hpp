@@ -106,10 +106,11 @@ struct MultiRun {
 struct PinnedStage {
     uint8_t *p = nullptr;
     size_t cap = 0;
+    std::vector<uint8_t *> retired;   // outgrown buffers, freed with the stage
     PinnedStage() = default;
     PinnedStage(const PinnedStage &) = delete;
     PinnedStage &operator=(const PinnedStage &) = delete;
-    PinnedStage(PinnedStage &&o) noexcept : p(o.p), cap(o.cap)
+    PinnedStage(PinnedStage &&o) noexcept : p(o.p), cap(o.cap), retired(std::move(o.retired))
     {
         o.p = nullptr;
         o.cap = 0;
@@ -122,13 +123,16 @@ struct MultiTune {
     uint32_t waves_per_group = 0;     // 1, 2 or 4
     uint32_t min_waves_per_cu = 0;    // plan for exactly this many waves a CU's LDS holds
     double alone = 0;                 // the issue fraction of a wave alone on a SIMD
+    uint32_t max_waves_per_cu = 0;    // resident waves a CU the launch may take (room
+                                      // left for other streams' kernels)
 };
 
 // Plan (widths, LDS region, grid) and enqueue one k_seq_multi launch on
 // ex's stream; dlflags (device, one byte a position: at most the parts'
 // lanes) or NULL.  Every part's program must be wide-eligible and of one
 // format.  The tables go up through `pin` when given, else from pageable
-// memory.  Does not wait.
+// memory; with `pin` the caller keeps launches in flight, so arenas grow
+// without waiting (arena_grow).  Does not wait.
 cft_status multi_launch(cft_device *ex, std::vector<MultiPart> &parts, uint8_t *dlflags,
                         const MultiTune &tune, MultiRun &run, const char *who,
                         PinnedStage *pin = nullptr);

@@ -203,6 +203,7 @@ typedef struct cftx_server_stats {
     uint64_t peak_bytes;     /* most device memory admitted at once */
     uint64_t slices;         /* program slices launched (within program_batches) */
     uint64_t max_slice_lanes;
+    double   slice_host_s;   /* scheduler time spent forming and launching slices */
 } cftx_server_stats;
 
 /* workers: streams (1..8, 0 for 4); hbm_budget: device bytes jobs may hold
