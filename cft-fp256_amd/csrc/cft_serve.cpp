@@ -887,9 +887,10 @@ extern "C" CFT_API cft_status cftx_server_open(cft_device *dev, int workers, siz
         w.index = k;
         w.can = workers == 1 ? (kCanElem | kCanProg) : (k < n_elem ? kCanElem : kCanProg);
         w.first_prog = workers == 1 || k == n_elem;
-        // elementwise workers on hardware queues of their own: their short
-        // launches never queue behind a program slice
-        st = device_open(dev->hip_index, dev->scratch_depth, &w.dev, w.can == kCanElem);
+        // every worker on a hardware queue of its own: an elementwise launch
+        // never queues behind a program slice, and two program workers'
+        // slices run side by side whatever other streams the process holds
+        st = device_open(dev->hip_index, dev->scratch_depth, &w.dev, true);
         hipError_t e = hipSuccess;
         if (st == CFT_OK) e = hipEventCreateWithFlags(&w.ev, hipEventDisableTiming);
         for (Slot &sl : w.slots) {
