@@ -206,6 +206,9 @@ def _bind(lib):
         "cft_run": ([V, c_int, c_int, c_int, V, V, V, V, c_size_t, P, P], c_int),
         "cft_run_ex": ([V, c_int, c_int, c_int, POINTER(CftElemArgs)], c_int),
         "cft_div": ([V, c_int, c_int, V, V, V, c_size_t, P, P], c_int),
+        "cft_reduce": ([V, c_int, c_int, c_int, V, V, V, c_size_t, P, P], c_int),
+        "cft_reduce_seg": ([V, c_int, c_int, c_int, V, V, V, c_size_t, c_size_t, P, P],
+                           c_int),
         "cft_sqrt": ([V, c_int, c_int, V, V, c_size_t, P, P], c_int),
         "cft_alloc": ([V, c_size_t, POINTER(c_void_p)], c_int),
         "cft_buffer_data": ([V], c_void_p),

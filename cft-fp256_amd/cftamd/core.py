@@ -101,6 +101,25 @@ class Device:
                              None), "cft_sqrt")
         return out, fl.value
 
+    # ---- reductions (host/include/cft.h:1210-1340) ---------------------
+    def reduce(self, op, fmt, rnd, a, b=None, out=None, n=None, seg=None):
+        """The reduction tree over a (and b for CFT_DOT): one result, or one
+        per segment of `seg` elements; returns (d, flags)."""
+        esz = _lib.FORMAT_BYTES.get(fmt, 1)          # the library refuses a bad format
+        if n is None:
+            n = (a.nbytes if isinstance(a, np.ndarray) else a.bytes) // esz
+        nres = 1 if seg is None else (n // seg if seg else 0)
+        if out is None:
+            out = np.zeros(max(nres, 1) * esz, dtype=np.uint8)
+        fl = c_uint32()
+        if seg is None:
+            check(lib().cft_reduce(self.handle, op, fmt, rnd, _ptr(a), _ptr(b), _ptr(out), n,
+                                   byref(fl), None), "cft_reduce")
+        else:
+            check(lib().cft_reduce_seg(self.handle, op, fmt, rnd, _ptr(a), _ptr(b), _ptr(out),
+                                       n, seg, byref(fl), None), "cft_reduce_seg")
+        return out, fl.value
+
     # ---- the 754 status word ------------------------------------------
     def save_all_flags(self):
         return lib().cft_save_all_flags(self.handle)

@@ -591,15 +591,14 @@ extern "C" CFT_API cft_status cft_get_caps(cft_device *dev, cft_caps *out)
     return CFT_OK;
 }
 
-// Every elementwise opcode and the four formats; the reductions (24, 25, 28,
-// 29, 31) are not on this engine's path.
+// Every assigned opcode in the four formats: the elementwise ones through
+// cft_run, the reductions (24, 25, 28, 29, 31) through cft_reduce.
 extern "C" CFT_API int cft_supports(cft_device *dev, cft_op op, cft_format fmt)
 {
     const int o = (int)op;
     if (!dev || (int)fmt < 0 || (int)fmt > 3)
         return 0;
-    const bool assigned = (o >= 0 && o <= 14) || (o >= 16 && o <= 23) || o == 26 || o == 27 ||
-                          o == 30;
+    const bool assigned = (o >= 0 && o <= 14) || (o >= 16 && o <= 31);
     return assigned ? 1 : 0;
 }
 

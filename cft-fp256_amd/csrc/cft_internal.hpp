@@ -24,7 +24,7 @@ constexpr uint32_t kSeqFeatures =
     CFT_SEQ_FEAT_WIDE_CONST | CFT_SEQ_FEAT_REGS32 | CFT_SEQ_FEAT_BANK_PTR | CFT_SEQ_FEAT_KX9 |
     CFT_ALU_EXT_IMUL | CFT_SEQ_FEAT_SCRATCH | CFT_SEQ_FEAT_SCRATCH_IO |
     CFT_SEQ_FEAT_SCRATCH_STRICT | CFT_SEQ_FEAT_SCALAR | CFT_SEQ_FEAT_LANE_MASK |
-    CFT_SEQ_FEAT_INDEXED |
+    CFT_SEQ_FEAT_INDEXED | CFT_FEAT_REDUCE_SEG |
     CFT_SEQ_FEAT_SCRATCH_STEP | CFT_SEQ_FEAT_LANE_FLAGS | CFT_SEQ_FEAT_FLAG_CONTROL;
 }
 
@@ -35,6 +35,7 @@ enum Role {
     ROLE_BANK, ROLE_SIN, ROLE_SOUT, ROLE_COUNTS, ROLE_MASK, ROLE_LFLAGS,
     ROLE_POOL, ROLE_TRIPS, ROLE_ORDER, ROLE_REGS,
     ROLE_MPROGS,                                    // the multi-program runner's tables
+    ROLE_RED0, ROLE_RED1, ROLE_RWORDS,              // a reduction's partials and flag words
     ROLE_COUNT
 };
 

@@ -1,0 +1,224 @@
+// The reductions' tree (cft_reduce, cft_reduce_seg), a few levels a launch.
+//
+// The contract's tree (host/include/cft.h:1210-1283, host/src/softfloat.c:
+// 880-925) splits a range so its left child is the largest power of two
+// strictly inside it.  That is the tree of pairing neighbours level by
+// level, an odd last node carried up unchanged: at every level the nodes
+// are the aligned power-of-two ranges, and the last, shorter one is the same
+// tree over its own length.  So is any grouping of aligned runs of 2^k
+// consecutive nodes: a launch takes each segment's partials (its elements,
+// the first time) in groups of four, writes one node a group, and the next
+// launch does the same over those, until one node a segment remains.  The
+// result is node for node the recursive tree's, whatever the segment length.
+//
+// The first launch also forms the leaves: the element (SUM, MAXALL), its
+// rounded product with b (DOT, SUMSQ) or its magnitude (SUMABS), and for
+// sumSquare / sumAbs it records, a segment, whether an infinity or a
+// signalling NaN was seen (754-2019 9.4's override, applied by the host).
+// Flags are ORed a segment.
+#include "fx_arith.hpp"
+#include "fx_misc.hpp"
+#include "kernels.h"
+
+using namespace fx;
+
+namespace {
+
+enum { LEAF_VALUE = 0, LEAF_MUL = 1, LEAF_ABS = 2 };
+enum { NODE_ADD = 0, NODE_MAX = 1 };
+
+template <int NL>
+__device__ __forceinline__ void ld(const uint32_t *__restrict__ p, uint64_t i, uint32_t (&x)[NL])
+{
+    if constexpr (NL == 8) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(p + i * 8);
+        const uint4 v0 = q[0], v1 = q[1];
+        x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
+        x[4] = v1.x; x[5] = v1.y; x[6] = v1.z; x[7] = v1.w;
+    } else if constexpr (NL == 4) {
+        const uint4 v0 = reinterpret_cast<const uint4 *>(p)[i];
+        x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
+    } else if constexpr (NL == 2) {
+        const uint2 v0 = reinterpret_cast<const uint2 *>(p)[i];
+        x[0] = v0.x; x[1] = v0.y;
+    } else {
+        x[0] = p[i];
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void st(uint32_t *__restrict__ p, uint64_t i, const uint32_t (&x)[NL])
+{
+    if constexpr (NL == 8) {
+        uint4 *q = reinterpret_cast<uint4 *>(p + i * 8);
+        q[0] = make_uint4(x[0], x[1], x[2], x[3]);
+        q[1] = make_uint4(x[4], x[5], x[6], x[7]);
+    } else if constexpr (NL == 4) {
+        reinterpret_cast<uint4 *>(p)[i] = make_uint4(x[0], x[1], x[2], x[3]);
+    } else if constexpr (NL == 2) {
+        reinterpret_cast<uint2 *>(p)[i] = make_uint2(x[0], x[1]);
+    } else {
+        p[i] = x[0];
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void pick(bool s, const uint32_t (&x)[NL], const uint32_t (&y)[NL],
+                                     uint32_t (&o)[NL])
+{
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        o[i] = s ? x[i] : y[i];
+}
+
+// 9.4's classes of an encoding: 1 an infinity, 2 a signalling NaN.
+template <int NL>
+__device__ __forceinline__ uint32_t special_bits(const uint32_t (&x)[NL])
+{
+    using Fm = F<NL>;
+    const uint32_t top = x[NL - 1];
+    if (((top >> Fm::TOPF) & Fm::EMASK) != Fm::EMASK)
+        return 0;
+    uint32_t frac = top & Fm::FMASK;
+#pragma unroll
+    for (int i = 0; i < NL - 1; i++)
+        frac |= x[i];
+    if (!frac)
+        return 1u;
+    return ((top >> (Fm::TOPF - 1)) & 1u) ? 0u : 2u;
+}
+
+// Bits into a segment's word: one atomic a wavefront where its lanes share
+// the segment, and only for bits the word does not hold yet.
+__device__ __forceinline__ void seg_or(uint32_t *words, uint64_t s, uint32_t bits)
+{
+    const uint64_t s0 = __builtin_amdgcn_readfirstlane((uint32_t)s) |
+                        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(s >> 32)) << 32);
+    if (__ballot(s != s0) == 0) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int b = 0; b < 5; b++)
+            if (__ballot((bits >> b) & 1u))
+                w |= 1u << b;
+        const uint64_t live = __ballot(1);
+        if (w && __lane_id() == (unsigned)__builtin_ctzll(live)) {
+            const uint32_t have = __hip_atomic_load(words + s0, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+            if (w & ~have)
+                atomicOr(words + s0, w);
+        }
+    } else if (bits) {
+        const uint32_t have = __hip_atomic_load(words + s, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        if (bits & ~have)
+            atomicOr(words + s, bits);
+    }
+}
+
+template <int NL>
+__global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src,
+                                                const uint32_t *__restrict__ srcb,
+                                                uint32_t *__restrict__ dst, uint64_t nseg,
+                                                uint64_t m, uint64_t m2, int leaf, int kind,
+                                                int rnd, int per_seg,
+                                                uint32_t *__restrict__ segflags,
+                                                uint32_t *__restrict__ segspec)
+{
+    const uint64_t total = nseg * m2;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += stride) {
+        const uint64_t s = o / m2, g = o - s * m2;
+        const uint64_t base = s * m + 4 * g;
+        const uint64_t left = m - 4 * g;
+        const uint32_t cnt = left < 4 ? (uint32_t)left : 4u;
+        uint32_t v0[NL] = {}, v1[NL] = {}, v2[NL] = {}, v3[NL] = {};
+        uint32_t fl = 0, spec = 0;
+        // the leaves (or the last launch's nodes); absent ones stay +0 unread
+#pragma unroll 1
+        for (uint32_t k = 0; k < cnt; k++) {
+            uint32_t x[NL];
+            ld<NL>(src, base + k, x);
+            if (segspec)
+                spec |= special_bits<NL>(x);
+            if (leaf == LEAF_MUL) {
+                uint32_t y[NL], p[NL];
+                ld<NL>(srcb, base + k, y);
+                fma_core<NL>(OP_MUL, x, y, x, rnd, p, fl);       // c unread
+#pragma unroll
+                for (int i = 0; i < NL; i++)
+                    x[i] = p[i];
+            } else if (leaf == LEAF_ABS) {
+                x[NL - 1] &= 0x7FFFFFFFu;                        // 5.5.1: no flags
+            }
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                v0[i] = k == 0 ? x[i] : v0[i];
+                v1[i] = k == 1 ? x[i] : v1[i];
+                v2[i] = k == 2 ? x[i] : v2[i];
+                v3[i] = k == 3 ? x[i] : v3[i];
+            }
+        }
+        // three nodes at most: (v0, v1), (v2, v3), then the two of those; an
+        // odd last one goes up as it is
+        uint32_t t0[NL], t1[NL];
+#pragma unroll
+        for (int i = 0; i < NL; i++) {
+            t0[i] = v0[i];
+            t1[i] = v2[i];
+        }
+#pragma unroll 1
+        for (int step = 0; step < 3; step++) {
+            const bool on = step == 0 ? cnt >= 2 : step == 1 ? cnt == 4 : cnt >= 3;
+            if (!on)
+                continue;
+            uint32_t x[NL], y[NL], r[NL];
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                x[i] = step == 0 ? v0[i] : step == 1 ? v2[i] : t0[i];
+                y[i] = step == 0 ? v1[i] : step == 1 ? v3[i] : t1[i];
+            }
+            if (kind == NODE_ADD)
+                fma_core<NL>(OP_ADD, x, x, y, rnd, r, fl);        // x * 1 + y
+            else
+                misc_op<NL>(8, x, y, y, r, fl);                    // 754-2019 maximum
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                t0[i] = step != 1 ? r[i] : t0[i];
+                t1[i] = step == 1 ? r[i] : t1[i];
+            }
+        }
+        st<NL>(dst, o, t0);
+        seg_or(segflags, per_seg ? s : 0, fl);
+        if (segspec)
+            seg_or(segspec, s, spec);
+    }
+}
+
+template <int NL>
+hipError_t launch(const void *src, const void *srcb, void *dst, uint64_t nseg, uint64_t m,
+                  uint64_t m2, int leaf, int kind, int rnd, int per_seg, uint32_t *segflags,
+                  uint32_t *segspec, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_reduce<NL>), dim3(grid), dim3(256), 0, st,
+                       (const uint32_t *)src, (const uint32_t *)srcb, (uint32_t *)dst, nseg, m,
+                       m2, leaf, kind, rnd, per_seg, segflags, segspec);
+    return hipGetLastError();
+}
+
+} // namespace
+
+extern "C" hipError_t cftk_reduce(int fmt, const void *src, const void *srcb, void *dst,
+                                  uint64_t nseg, uint64_t m, uint64_t m2, int leaf, int kind,
+                                  int rnd, int per_seg, uint32_t *segflags, uint32_t *segspec,
+                                  int grid, hipStream_t st)
+{
+#define CFT_REDUCE_ARGS src, srcb, dst, nseg, m, m2, leaf, kind, rnd, per_seg, segflags, segspec, \
+                        grid, st
+    switch (fmt) {
+    case 0: return launch<1>(CFT_REDUCE_ARGS);
+    case 1: return launch<2>(CFT_REDUCE_ARGS);
+    case 2: return launch<4>(CFT_REDUCE_ARGS);
+    default: return launch<8>(CFT_REDUCE_ARGS);
+    }
+#undef CFT_REDUCE_ARGS
+}

@@ -21,6 +21,18 @@ hipError_t cftk_elem(int fmt, int op, int rnd, const void *a, const void *b, con
 hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *b, void *d,
                         size_t n, uint32_t smask, uint32_t *flags, int grid, hipStream_t st);
 
+// One launch of the reductions' tree: for each of nseg segments of m
+// partials (segment s at src[s*m ..)), groups of four consecutive partials
+// become one node at dst[s*m2 + g], m2 = ceil(m/4).  leaf (the first launch
+// only): 0 the partial itself, 1 its rounded product with srcb's, 2 its
+// magnitude; kind: 0 ADD nodes, 1 maximum nodes.  Flags are ORed into
+// segflags[s] (per_seg) or segflags[0]; segspec, when given, gets bit 0 where
+// a partial of the segment is an infinity and bit 1 where it is a signalling
+// NaN.
+hipError_t cftk_reduce(int fmt, const void *src, const void *srcb, void *dst, uint64_t nseg,
+                       uint64_t m, uint64_t m2, int leaf, int kind, int rnd, int per_seg,
+                       uint32_t *segflags, uint32_t *segspec, int grid, hipStream_t st);
+
 // p[0..n) = v
 hipError_t cftk_fill32(uint32_t *p, uint32_t v, size_t n, hipStream_t st);
 

@@ -152,6 +152,20 @@ CFT_API cft_status cft_run(cft_device *dev, cft_op op, cft_format fmt,
                            const void *c, void *d, size_t n,
                            uint32_t *flags_out, uint32_t *bus_out);
 
+/* ---- reductions: reference host/include/cft.h:1210-1340 ---------------
+ * d receives one element (cft_reduce) or n / seg elements (cft_reduce_seg):
+ * the fixed tree whose left child is the largest power of two strictly
+ * inside the range, at the call's rounding attribute at every node.
+ * Opcodes CFT_SUM, CFT_DOT, CFT_SUMSQ, CFT_SUMABS, CFT_MAXALL. */
+CFT_API cft_status cft_reduce(cft_device *dev, cft_op op, cft_format fmt,
+                              cft_round rnd, const void *a, const void *b,
+                              void *d, size_t n, uint32_t *flags_out,
+                              uint32_t *bus_out);
+CFT_API cft_status cft_reduce_seg(cft_device *dev, cft_op op, cft_format fmt,
+                                  cft_round rnd, const void *a, const void *b,
+                                  void *d, size_t n, size_t seg,
+                                  uint32_t *flags_out, uint32_t *bus_out);
+
 /* reference host/include/cft.h:1385-1402 */
 CFT_API cft_status cft_div(cft_device *dev, cft_format fmt, cft_round rnd,
                            const void *a, const void *b, void *d, size_t n,

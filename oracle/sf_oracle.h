@@ -14,6 +14,8 @@
  *                      python/cft_golden/softfloat.py:440-506 (read as text)
  *   elementwise call   host/src/device.c:1699-2174 (NULL operand = +0,
  *                      scalar operand = element 0, flags OR)
+ *   reductions         host/src/softfloat.c:858-925, host/src/device.c:
+ *                      2678-3062 and 3219-3400 (rd_oracle.c)
  *
  * The arithmetic is our own: exact integer arithmetic on a fixed-size
  * little-endian bignum, with an alignment bound that substitutes a far
@@ -65,6 +67,15 @@ int oc_sqrt(int fmt, int rnd, const uint8_t *a, uint8_t *out, uint32_t *flags);
 int oc_run(int fmt, int op, int rnd, const void *a, const void *b,
            const void *c, void *d, size_t n, uint32_t scalar_mask,
            int threads, uint32_t *flags_out);
+
+/* The reductions (rd_oracle.c): cft_reduce's one result of op (24 SUM,
+ * 25 DOT, 28 SUMSQ, 29 SUMABS, 31 MAXALL) over n elements, and
+ * cft_reduce_seg's n / seg results.  b is read by DOT only.  Returns 1 for
+ * an argument the entry points refuse. */
+int oc_reduce(int fmt, int op, int rnd, const void *a, const void *b, void *d, size_t n,
+              uint32_t *flags_out);
+int oc_reduce_seg(int fmt, int op, int rnd, const void *a, const void *b, void *d,
+                  size_t n, size_t seg, uint32_t *flags_out);
 
 /* Rounding a caller-built exact value: (-1)^sign * m * 2^e, m given as
  * little-endian bytes (mbytes of them).  Used by the tests' closed-form
