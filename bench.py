@@ -561,6 +561,44 @@ def main():
                          "kernel_ms": round(ks * 1e3, 4),
                          "kernel_GBs": round(BYTES[name] * n / ks / 1e9, 1)}
             log(f"bench: {name} {ops[name]}")
+        # reductions over the same resident binary256 operands: the tree over
+        # the whole array, and the same over segments (one result a segment)
+        red = {}
+        for name, op, seg, b_ptr in (("sum", 24, None, None),
+                                     ("dot", 25, None, bufs[1].data_ptr),
+                                     ("maxall", 31, None, None),
+                                     ("sum_seg4096", 24, 4096, None),
+                                     ("sum_seg16", 24, 16, None)):
+            nres = 1 if seg is None else n // seg
+            rbuf = dev.alloc(nres * 32)
+            f2 = ctypes.c_uint32()
+
+            def red_call():
+                if seg is None:
+                    st = lib.cft_reduce(dev.handle, op, 3, 0, bufs[0].data_ptr, b_ptr,
+                                        rbuf.data_ptr, n, ctypes.byref(f2), None)
+                else:
+                    st = lib.cft_reduce_seg(dev.handle, op, 3, 0, bufs[0].data_ptr, b_ptr,
+                                            rbuf.data_ptr, n, seg, ctypes.byref(f2), None)
+                assert st == 0, lib.cft_last_error()
+
+            red_call()
+            lib.cftx_kernel_timer(dev.handle, 1)
+            t1 = time.perf_counter()
+            for _ in range(5):
+                red_call()
+            dt = (time.perf_counter() - t1) / 5
+            lib.cftx_kernel_timer_read(dev.handle, ctypes.byref(kms), ctypes.byref(klaunch))
+            lib.cftx_kernel_timer(dev.handle, 0)
+            ks = kms.value / 1e3 / max(1, klaunch.value)
+            rbytes = (64 if op == 25 else 32) * n
+            red[name] = {"Gelem_s": round(n / dt / 1e9, 3), "kernel_ms": round(ks * 1e3, 4),
+                         "kernel_Gelem_s": round(n / ks / 1e9, 3),
+                         "kernel_GBs": round(rbytes / ks / 1e9, 1), "results": nres,
+                         "flags": f2.value}
+            log(f"bench: reduce {name} {red[name]}")
+            rbuf.free()
+        ops["reduce_fp256"] = red
         # the binary128 configuration: FMA RNE over 2^20 elements (seed 0),
         # checked against the reference CPU path on all of them
         n1 = 1 << 20

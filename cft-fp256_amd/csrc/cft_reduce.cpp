@@ -149,8 +149,6 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
         return st;
     if (op == OP_DOT && (st = bind_in(dev, ROLE_B, b, n * esz, &db, who)) != CFT_OK)
         return st;
-    if (op == OP_SUMSQ)
-        db = da;
     // the partials: the first launch writes nseg * ceil(seg/4), each later one
     // a quarter of the one before, ping-ponging between two arenas
     const size_t first = nseg * ((seg + 3) / 4);
@@ -168,7 +166,7 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
     hipError_t e = cftk_fill32(segflags, 0u, nwords, dev->stream);
     if (e == hipSuccess)
         e = timer_begin(dev);
-    const int leaf = op == OP_DOT || op == OP_SUMSQ ? 1 : op == OP_SUMABS ? 2 : 0;
+    const int leaf = op == OP_DOT ? 1 : op == OP_SUMABS ? 2 : op == OP_SUMSQ ? 3 : 0;
     const int kind = op == OP_MAXALL ? 1 : 0;
     size_t m = seg;
     const void *src = da;

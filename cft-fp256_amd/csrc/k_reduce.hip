@@ -12,7 +12,8 @@
 // result is node for node the recursive tree's, whatever the segment length.
 //
 // The first launch also forms the leaves: the element (SUM, MAXALL), its
-// rounded product with b (DOT, SUMSQ) or its magnitude (SUMABS), and for
+// rounded product with b (DOT) or itself (SUMSQ), or its magnitude (SUMABS);
+// each thread issues its group's loads before using any of them.  For
 // sumSquare / sumAbs it records, a segment, whether an infinity or a
 // signalling NaN was seen (754-2019 9.4's override, applied by the host).
 // Flags are ORed a segment.
@@ -24,7 +25,7 @@ using namespace fx;
 
 namespace {
 
-enum { LEAF_VALUE = 0, LEAF_MUL = 1, LEAF_ABS = 2 };
+enum { LEAF_VALUE = 0, LEAF_MUL = 1, LEAF_ABS = 2, LEAF_SQ = 3 };
 enum { NODE_ADD = 0, NODE_MAX = 1 };
 
 template <int NL>
@@ -60,15 +61,6 @@ __device__ __forceinline__ void st(uint32_t *__restrict__ p, uint64_t i, const u
     } else {
         p[i] = x[0];
     }
-}
-
-template <int NL>
-__device__ __forceinline__ void pick(bool s, const uint32_t (&x)[NL], const uint32_t (&y)[NL],
-                                     uint32_t (&o)[NL])
-{
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        o[i] = s ? x[i] : y[i];
 }
 
 // 9.4's classes of an encoding: 1 an infinity, 2 a signalling NaN.
@@ -115,47 +107,88 @@ __device__ __forceinline__ void seg_or(uint32_t *words, uint64_t s, uint32_t bit
     }
 }
 
-template <int NL>
+template <int NL, int LEAF, int KIND>
 __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src,
                                                 const uint32_t *__restrict__ srcb,
                                                 uint32_t *__restrict__ dst, uint64_t nseg,
-                                                uint64_t m, uint64_t m2, int leaf, int kind,
-                                                int rnd, int per_seg,
+                                                uint64_t m, uint64_t m2, uint64_t sq,
+                                                uint64_t sr, int rnd, int per_seg,
                                                 uint32_t *__restrict__ segflags,
                                                 uint32_t *__restrict__ segspec)
 {
     const uint64_t total = nseg * m2;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += stride) {
-        const uint64_t s = o / m2, g = o - s * m2;
-        const uint64_t base = s * m + 4 * g;
+    uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // (segment, group) of o, then walked by the stride's own (sq, sr): no
+    // division in the loop.  o < 2^32, and m2 <= o below, so 32 bits divide.
+    uint64_t s = 0, g = o;
+    if (o >= m2) {
+        s = (uint32_t)o / (uint32_t)m2;
+        g = o - s * m2;
+    }
+    uint64_t base = s * m + 4 * g;
+    const uint64_t dbase = sq * m + 4 * sr;
+    const uint64_t wrap = m - 4 * m2;            // s + 1, g - m2 (mod 2^64)
+    const bool by_seg = per_seg || segspec;
+    uint64_t cur = s;
+    uint32_t fl = 0, spec = 0;
+    for (; o < total; o += stride) {
+        if (by_seg && s != cur) {                // a new segment: the last one's bits out
+            seg_or(segflags, per_seg ? cur : 0, fl);
+            if (segspec)
+                seg_or(segspec, cur, spec);
+            fl = spec = 0;
+            cur = s;
+        }
         const uint64_t left = m - 4 * g;
         const uint32_t cnt = left < 4 ? (uint32_t)left : 4u;
+        // the group's partials, every load issued before the first is used;
+        // absent ones stay +0 and are never combined
         uint32_t v0[NL] = {}, v1[NL] = {}, v2[NL] = {}, v3[NL] = {};
-        uint32_t fl = 0, spec = 0;
-        // the leaves (or the last launch's nodes); absent ones stay +0 unread
+        ld<NL>(src, base, v0);
+        if (cnt > 1)
+            ld<NL>(src, base + 1, v1);
+        if (cnt > 2)
+            ld<NL>(src, base + 2, v2);
+        if (cnt > 3)
+            ld<NL>(src, base + 3, v3);
+        if (segspec)
+            spec |= special_bits<NL>(v0) | special_bits<NL>(v1) | special_bits<NL>(v2) |
+                    special_bits<NL>(v3);
+        if constexpr (LEAF == LEAF_ABS) {
+            v0[NL - 1] &= 0x7FFFFFFFu;                           // 5.5.1: no flags
+            v1[NL - 1] &= 0x7FFFFFFFu;
+            v2[NL - 1] &= 0x7FFFFFFFu;
+            v3[NL - 1] &= 0x7FFFFFFFu;
+        }
+        if constexpr (LEAF == LEAF_MUL || LEAF == LEAF_SQ) {
+            // the products in place, one multiply body: each turn multiplies
+            // v0 and rotates it to the back
 #pragma unroll 1
-        for (uint32_t k = 0; k < cnt; k++) {
-            uint32_t x[NL];
-            ld<NL>(src, base + k, x);
-            if (segspec)
-                spec |= special_bits<NL>(x);
-            if (leaf == LEAF_MUL) {
-                uint32_t y[NL], p[NL];
-                ld<NL>(srcb, base + k, y);
-                fma_core<NL>(OP_MUL, x, y, x, rnd, p, fl);       // c unread
+            for (uint32_t k = 0; k < 4; k++) {
+                uint32_t r[NL];
+                if (k < cnt) {
+                    uint32_t y[NL];
+                    if constexpr (LEAF == LEAF_MUL) {
+                        ld<NL>(srcb, base + k, y);
+                    } else {
 #pragma unroll
-                for (int i = 0; i < NL; i++)
-                    x[i] = p[i];
-            } else if (leaf == LEAF_ABS) {
-                x[NL - 1] &= 0x7FFFFFFFu;                        // 5.5.1: no flags
-            }
+                        for (int i = 0; i < NL; i++)
+                            y[i] = v0[i];
+                    }
+                    fma_core<NL>(OP_MUL, v0, y, v0, rnd, r, fl);    // c unread
+                } else {
 #pragma unroll
-            for (int i = 0; i < NL; i++) {
-                v0[i] = k == 0 ? x[i] : v0[i];
-                v1[i] = k == 1 ? x[i] : v1[i];
-                v2[i] = k == 2 ? x[i] : v2[i];
-                v3[i] = k == 3 ? x[i] : v3[i];
+                    for (int i = 0; i < NL; i++)
+                        r[i] = v0[i];
+                }
+#pragma unroll
+                for (int i = 0; i < NL; i++) {
+                    v0[i] = v1[i];
+                    v1[i] = v2[i];
+                    v2[i] = v3[i];
+                    v3[i] = r[i];
+                }
             }
         }
         // three nodes at most: (v0, v1), (v2, v3), then the two of those; an
@@ -177,7 +210,7 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
                 x[i] = step == 0 ? v0[i] : step == 1 ? v2[i] : t0[i];
                 y[i] = step == 0 ? v1[i] : step == 1 ? v3[i] : t1[i];
             }
-            if (kind == NODE_ADD)
+            if constexpr (KIND == NODE_ADD)
                 fma_core<NL>(OP_ADD, x, x, y, rnd, r, fl);        // x * 1 + y
             else
                 misc_op<NL>(8, x, y, y, r, fl);                    // 754-2019 maximum
@@ -188,10 +221,30 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
             }
         }
         st<NL>(dst, o, t0);
-        seg_or(segflags, per_seg ? s : 0, fl);
-        if (segspec)
-            seg_or(segspec, s, spec);
+        s += sq;
+        g += sr;
+        base += dbase;
+        if (g >= m2) {
+            g -= m2;
+            s++;
+            base += wrap;
+        }
     }
+    seg_or(segflags, per_seg ? cur : 0, fl);
+    if (segspec)
+        seg_or(segspec, cur, spec);
+}
+
+template <int NL, int LEAF, int KIND>
+hipError_t launch_one(const void *src, const void *srcb, void *dst, uint64_t nseg, uint64_t m,
+                      uint64_t m2, int rnd, int per_seg, uint32_t *segflags, uint32_t *segspec,
+                      int grid, hipStream_t st)
+{
+    const uint64_t stride = (uint64_t)grid * 256;
+    hipLaunchKernelGGL((k_reduce<NL, LEAF, KIND>), dim3(grid), dim3(256), 0, st,
+                       (const uint32_t *)src, (const uint32_t *)srcb, (uint32_t *)dst, nseg, m,
+                       m2, stride / m2, stride % m2, rnd, per_seg, segflags, segspec);
+    return hipGetLastError();
 }
 
 template <int NL>
@@ -199,10 +252,16 @@ hipError_t launch(const void *src, const void *srcb, void *dst, uint64_t nseg, u
                   uint64_t m2, int leaf, int kind, int rnd, int per_seg, uint32_t *segflags,
                   uint32_t *segspec, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_reduce<NL>), dim3(grid), dim3(256), 0, st,
-                       (const uint32_t *)src, (const uint32_t *)srcb, (uint32_t *)dst, nseg, m,
-                       m2, leaf, kind, rnd, per_seg, segflags, segspec);
-    return hipGetLastError();
+#define CFT_ONE_ARGS src, srcb, dst, nseg, m, m2, rnd, per_seg, segflags, segspec, grid, st
+    if (kind == NODE_MAX)
+        return launch_one<NL, LEAF_VALUE, NODE_MAX>(CFT_ONE_ARGS);
+    switch (leaf) {
+    case LEAF_MUL: return launch_one<NL, LEAF_MUL, NODE_ADD>(CFT_ONE_ARGS);
+    case LEAF_SQ: return launch_one<NL, LEAF_SQ, NODE_ADD>(CFT_ONE_ARGS);
+    case LEAF_ABS: return launch_one<NL, LEAF_ABS, NODE_ADD>(CFT_ONE_ARGS);
+    default: return launch_one<NL, LEAF_VALUE, NODE_ADD>(CFT_ONE_ARGS);
+    }
+#undef CFT_ONE_ARGS
 }
 
 } // namespace

@@ -25,7 +25,7 @@ hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *
 // partials (segment s at src[s*m ..)), groups of four consecutive partials
 // become one node at dst[s*m2 + g], m2 = ceil(m/4).  leaf (the first launch
 // only): 0 the partial itself, 1 its rounded product with srcb's, 2 its
-// magnitude; kind: 0 ADD nodes, 1 maximum nodes.  Flags are ORed into
+// magnitude, 3 its rounded square; kind: 0 ADD nodes, 1 maximum nodes.  Flags are ORed into
 // segflags[s] (per_seg) or segflags[0]; segspec, when given, gets bit 0 where
 // a partial of the segment is an infinity and bit 1 where it is a signalling
 // NaN.
