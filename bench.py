@@ -23,7 +23,10 @@ Beside it, in the same JSON line:
               p50/p99 latency, throughput, every job checked against the job
               run on its own, the plain one-job-a-launch form on a prefix;
   cpu_baseline  the reference's CPU FMA on this host's cores, same inputs,
-              compared bit for bit.
+              compared bit for bit;
+  roofline    the FMA kernel against 8 TB/s, and against the rate a library
+              ternary elementwise kernel (torch.addcmul) streams over the same
+              bytes (stream_GBs, frac_of_stream).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --copy-calibration BYTES     (a plain device copy, for
@@ -122,6 +125,34 @@ def copy_calibration(nbytes):
         y.copy_(x)
     torch.cuda.synchronize()
     log(f"bench: copied {nbytes} bytes three times")
+
+
+def stream_probe(n_elem, elem_bytes=32, reps=5):
+    """The streaming rate a library ternary elementwise kernel reaches on the
+    FMA's byte pattern: torch.addcmul over float32 tensors of n_elem *
+    elem_bytes bytes each, three read and one written, timed with events on
+    torch's stream.  The practical ceiling beside the 8 TB/s peak."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        k = n_elem * elem_bytes // 4
+        a, b, c = (torch.rand(k, device="cuda") for _ in range(3))
+        d = torch.empty_like(a)
+        torch.addcmul(c, a, b, out=d)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            torch.addcmul(c, a, b, out=d)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del a, b, c, d
+        torch.cuda.empty_cache()
+        return 4 * n_elem * elem_bytes / (ms * 1e-3) / 1e9
+    except Exception as e:                       # a measurement aid, never the result
+        log(f"bench: stream probe skipped ({e})")
+        return None
 
 
 def ref_program_chain(img, bank, state, steps, fmt, threads):
@@ -655,6 +686,10 @@ def main():
             bit_exact = mism == 0
             log(f"bench: CPU {cpu['value']} Gelem/s ({cpu['kind']}), mismatches {mism}")
 
+    stream = stream_probe(n)
+    if stream:
+        log(f"bench: torch.addcmul on the same bytes streams {stream:.0f} GB/s; the FMA kernel "
+            f"moves {achieved:.0f} GB/s ({achieved / stream:.2f} of it)")
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -676,7 +711,9 @@ def main():
                      "traffic": round(FMA_TRAFFIC_PER_ELEM * n),
                      "kernel_ms": round(kavg_s * 1e3, 4),
                      "kernel_Gelem_s": round(n / kavg_s / 1e9, 4),
-                     "bytes_per_elem": FMA_BYTES},
+                     "bytes_per_elem": FMA_BYTES,
+                     "stream_GBs": round(stream, 1) if stream else None,
+                     "frac_of_stream": round(achieved / stream, 4) if stream else None},
         "cpu_baseline": cpu,
         "bit_exact_vs_cpu": bit_exact,
         "ops": ops,

@@ -4,9 +4,10 @@
     python -m cftamd.run_workload l63   [--seed S]           Lorenz-63 ensemble, one launch
     python -m cftamd.run_workload l96   [--fmt 1|3] [--seed S]  Lorenz-96, one launch
     python -m cftamd.run_workload serve [--seconds T] [--seed S]  the serving stream
-    python -m cftamd.run_workload reduce [--n N] [--fmt 2|3] [--seg S] [--seed S]
-                                        a sum over the FMA operands (binary128 or
-                                        binary256), 5 modes
+    python -m cftamd.run_workload reduce [--n N] [--fmt 2|3] [--seg S] [--op OP] [--seed S]
+                                        a reduction (SUM 24, DOT 25, SUMSQ 28,
+                                        SUMABS 29, MAXALL 31) over the FMA
+                                        operands (binary128 or binary256), 5 modes
 
 Each builds its inputs as bench.py does (same generators and seeds), runs
 the workload once on GPU 0 with every input resident in device memory, and
@@ -52,20 +53,22 @@ def reduce(args):
         workload.elementwise(1 if fmt == 2 else 2, n, seed=args.seed, threads=16,
                              out=(a.host, scratch[0].host, scratch[1].host))
         a.to_device()
+        scratch[0].to_device()
+        bptr = scratch[0].data_ptr if args.op == 25 else None
         nres = n // args.seg if args.seg else 1
         d = dev.alloc(nres * esz)
         fl = ctypes.c_uint32()
         t0 = time.perf_counter()
         for rnd in range(5):
             if args.seg:
-                st = lib.cft_reduce_seg(dev.handle, 24, fmt, rnd, a.data_ptr, None, d.data_ptr,
-                                        n, args.seg, ctypes.byref(fl), None)
+                st = lib.cft_reduce_seg(dev.handle, args.op, fmt, rnd, a.data_ptr, bptr,
+                                        d.data_ptr, n, args.seg, ctypes.byref(fl), None)
             else:
-                st = lib.cft_reduce(dev.handle, 24, fmt, rnd, a.data_ptr, None, d.data_ptr, n,
-                                    ctypes.byref(fl), None)
+                st = lib.cft_reduce(dev.handle, args.op, fmt, rnd, a.data_ptr, bptr, d.data_ptr,
+                                    n, ctypes.byref(fl), None)
             _lib.check(st, "cft_reduce")
         dt = time.perf_counter() - t0
-        print(f"reduce fmt {fmt} seg {args.seg}: {5 * n} elements in {dt:.4f} s "
+        print(f"reduce op {args.op} fmt {fmt} seg {args.seg}: {5 * n} elements in {dt:.4f} s "
               f"({5 * n / dt / 1e9:.3f} Gelem/s), flags {fl.value}")
         for b in [a, d] + scratch:
             b.free()
@@ -126,6 +129,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--fmt", type=int, default=3, choices=[1, 2, 3])
     ap.add_argument("--seg", type=int, default=0)
+    ap.add_argument("--op", type=int, default=24, choices=[24, 25, 28, 29, 31])
     ap.add_argument("--seconds", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)

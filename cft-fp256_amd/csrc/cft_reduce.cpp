@@ -149,10 +149,10 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
         return st;
     if (op == OP_DOT && (st = bind_in(dev, ROLE_B, b, n * esz, &db, who)) != CFT_OK)
         return st;
-    // the partials: the first launch writes nseg * ceil(seg/4), each later one
-    // a quarter of the one before, ping-ponging between two arenas
-    const size_t first = nseg * ((seg + 3) / 4);
-    const size_t second = nseg * ((((seg + 3) / 4) + 3) / 4);
+    // the partials: the first launch writes nseg * ceil(seg/16), each later
+    // one a sixteenth of the one before, ping-ponging between two arenas
+    const size_t first = nseg * ((seg + 15) / 16);
+    const size_t second = nseg * ((((seg + 15) / 16) + 15) / 16);
     if ((st = arena_reserve(dev, dev->stage[ROLE_RED0], first * esz, who)) != CFT_OK ||
         (st = arena_reserve(dev, dev->stage[ROLE_RED1], second * esz, who)) != CFT_OK)
         return st;
@@ -172,7 +172,7 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
     const void *src = da;
     int out = 0;
     for (int pass = 0; e == hipSuccess; pass++) {
-        const size_t m2 = (m + 3) / 4;
+        const size_t m2 = (m + 15) / 16;
         e = cftk_reduce(fmt, src, pass == 0 ? db : nullptr, part[out], nseg, m, m2,
                         pass == 0 ? leaf : 0, kind, rnd, override94 ? 1 : 0, segflags,
                         pass == 0 ? segspec : nullptr, grid_for(dev, nseg * m2), dev->stream);

@@ -7,9 +7,11 @@
 // are the aligned power-of-two ranges, and the last, shorter one is the same
 // tree over its own length.  So is any grouping of aligned runs of 2^k
 // consecutive nodes: a launch takes each segment's partials (its elements,
-// the first time) in groups of four, writes one node a group, and the next
-// launch does the same over those, until one node a segment remains.  The
-// result is node for node the recursive tree's, whatever the segment length.
+// the first time) in groups of sixteen - four groups of four, each reduced
+// to a sub-tree, then the sub-trees as a group of four - writes one node a
+// group, and the next launch does the same over those, until one node a
+// segment remains.  The result is node for node the recursive tree's,
+// whatever the segment length.
 //
 // The first launch also forms the leaves: the element (SUM, MAXALL), its
 // rounded product with b (DOT) or itself (SUMSQ), or its magnitude (SUMABS);
@@ -126,9 +128,9 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
         s = (uint32_t)o / (uint32_t)m2;
         g = o - s * m2;
     }
-    uint64_t base = s * m + 4 * g;
-    const uint64_t dbase = sq * m + 4 * sr;
-    const uint64_t wrap = m - 4 * m2;            // s + 1, g - m2 (mod 2^64)
+    uint64_t base = s * m + 16 * g;
+    const uint64_t dbase = sq * m + 16 * sr;
+    const uint64_t wrap = m - 16 * m2;           // s + 1, g - m2 (mod 2^64)
     const bool by_seg = per_seg || segspec;
     uint64_t cur = s;
     uint32_t fl = 0, spec = 0;
@@ -140,84 +142,112 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
             fl = spec = 0;
             cur = s;
         }
-        const uint64_t left = m - 4 * g;
-        const uint32_t cnt = left < 4 ? (uint32_t)left : 4u;
-        // the group's partials, every load issued before the first is used;
-        // absent ones stay +0 and are never combined
-        uint32_t v0[NL] = {}, v1[NL] = {}, v2[NL] = {}, v3[NL] = {};
-        ld<NL>(src, base, v0);
-        if (cnt > 1)
-            ld<NL>(src, base + 1, v1);
-        if (cnt > 2)
-            ld<NL>(src, base + 2, v2);
-        if (cnt > 3)
-            ld<NL>(src, base + 3, v3);
-        if (segspec)
-            spec |= special_bits<NL>(v0) | special_bits<NL>(v1) | special_bits<NL>(v2) |
-                    special_bits<NL>(v3);
-        if constexpr (LEAF == LEAF_ABS) {
-            v0[NL - 1] &= 0x7FFFFFFFu;                           // 5.5.1: no flags
-            v1[NL - 1] &= 0x7FFFFFFFu;
-            v2[NL - 1] &= 0x7FFFFFFFu;
-            v3[NL - 1] &= 0x7FFFFFFFu;
-        }
-        if constexpr (LEAF == LEAF_MUL || LEAF == LEAF_SQ) {
-            // the products in place, one multiply body: each turn multiplies
-            // v0 and rotates it to the back
+        const uint64_t left = m - 16 * g;
+        const uint32_t cnt16 = left < 16 ? (uint32_t)left : 16u;
+        // rounds 0..3 reduce sub-group r (partials 4r .. 4r+3 of the group)
+        // to one sub-tree, rotated into u3; round 4 reduces the sub-trees
+        uint32_t u0[NL] = {}, u1[NL] = {}, u2[NL] = {}, u3[NL] = {};
+        uint32_t t0[NL], t1[NL];
 #pragma unroll 1
-            for (uint32_t k = 0; k < 4; k++) {
-                uint32_t r[NL];
-                if (k < cnt) {
-                    uint32_t y[NL];
-                    if constexpr (LEAF == LEAF_MUL) {
-                        ld<NL>(srcb, base + k, y);
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < NL; i++)
-                            y[i] = v0[i];
-                    }
-                    fma_core<NL>(OP_MUL, v0, y, v0, rnd, r, fl);    // c unread
-                } else {
-#pragma unroll
-                    for (int i = 0; i < NL; i++)
-                        r[i] = v0[i];
+        for (uint32_t rd = 0; rd < 5; rd++) {
+            uint32_t cnt;
+            uint32_t v0[NL] = {}, v1[NL] = {}, v2[NL] = {}, v3[NL] = {};
+            if (rd < 4) {
+                // the sub-group's partials, every load issued before the
+                // first is used; absent ones stay +0 and are never combined
+                cnt = cnt16 > 4 * rd ? min(cnt16 - 4 * rd, 4u) : 0u;
+                const uint64_t b4 = base + 4 * rd;
+                if (cnt > 0)
+                    ld<NL>(src, b4, v0);
+                if (cnt > 1)
+                    ld<NL>(src, b4 + 1, v1);
+                if (cnt > 2)
+                    ld<NL>(src, b4 + 2, v2);
+                if (cnt > 3)
+                    ld<NL>(src, b4 + 3, v3);
+                if (segspec)
+                    spec |= special_bits<NL>(v0) | special_bits<NL>(v1) |
+                            special_bits<NL>(v2) | special_bits<NL>(v3);
+                if constexpr (LEAF == LEAF_ABS) {
+                    v0[NL - 1] &= 0x7FFFFFFFu;                   // 5.5.1: no flags
+                    v1[NL - 1] &= 0x7FFFFFFFu;
+                    v2[NL - 1] &= 0x7FFFFFFFu;
+                    v3[NL - 1] &= 0x7FFFFFFFu;
                 }
+                if constexpr (LEAF == LEAF_MUL || LEAF == LEAF_SQ) {
+                    // the products in place, one multiply body: each turn
+                    // multiplies v0 and rotates it to the back
+#pragma unroll 1
+                    for (uint32_t k = 0; k < 4; k++) {
+                        uint32_t r[NL];
+                        if (k < cnt) {
+                            uint32_t y[NL];
+                            if constexpr (LEAF == LEAF_MUL) {
+                                ld<NL>(srcb, b4 + k, y);
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < NL; i++)
+                                    y[i] = v0[i];
+                            }
+                            fma_core<NL>(OP_MUL, v0, y, v0, rnd, r, fl);    // c unread
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < NL; i++)
+                                r[i] = v0[i];
+                        }
+#pragma unroll
+                        for (int i = 0; i < NL; i++) {
+                            v0[i] = v1[i];
+                            v1[i] = v2[i];
+                            v2[i] = v3[i];
+                            v3[i] = r[i];
+                        }
+                    }
+                }
+            } else {
+                cnt = (cnt16 + 3) / 4;
 #pragma unroll
                 for (int i = 0; i < NL; i++) {
-                    v0[i] = v1[i];
-                    v1[i] = v2[i];
-                    v2[i] = v3[i];
-                    v3[i] = r[i];
+                    v0[i] = u0[i];
+                    v1[i] = u1[i];
+                    v2[i] = u2[i];
+                    v3[i] = u3[i];
                 }
             }
-        }
-        // three nodes at most: (v0, v1), (v2, v3), then the two of those; an
-        // odd last one goes up as it is
-        uint32_t t0[NL], t1[NL];
-#pragma unroll
-        for (int i = 0; i < NL; i++) {
-            t0[i] = v0[i];
-            t1[i] = v2[i];
-        }
-#pragma unroll 1
-        for (int step = 0; step < 3; step++) {
-            const bool on = step == 0 ? cnt >= 2 : step == 1 ? cnt == 4 : cnt >= 3;
-            if (!on)
-                continue;
-            uint32_t x[NL], y[NL], r[NL];
+            // three nodes at most: (v0, v1), (v2, v3), then the two of
+            // those; an odd last one goes up as it is
 #pragma unroll
             for (int i = 0; i < NL; i++) {
-                x[i] = step == 0 ? v0[i] : step == 1 ? v2[i] : t0[i];
-                y[i] = step == 0 ? v1[i] : step == 1 ? v3[i] : t1[i];
+                t0[i] = v0[i];
+                t1[i] = v2[i];
             }
-            if constexpr (KIND == NODE_ADD)
-                fma_core<NL>(OP_ADD, x, x, y, rnd, r, fl);        // x * 1 + y
-            else
-                misc_op<NL>(8, x, y, y, r, fl);                    // 754-2019 maximum
+#pragma unroll 1
+            for (int step = 0; step < 3; step++) {
+                const bool on = step == 0 ? cnt >= 2 : step == 1 ? cnt == 4 : cnt >= 3;
+                if (!on)
+                    continue;
+                uint32_t x[NL], y[NL], r[NL];
+#pragma unroll
+                for (int i = 0; i < NL; i++) {
+                    x[i] = step == 0 ? v0[i] : step == 1 ? v2[i] : t0[i];
+                    y[i] = step == 0 ? v1[i] : step == 1 ? v3[i] : t1[i];
+                }
+                if constexpr (KIND == NODE_ADD)
+                    fma_core<NL>(OP_ADD, x, x, y, rnd, r, fl);    // x * 1 + y
+                else
+                    misc_op<NL>(8, x, y, y, r, fl);                // 754-2019 maximum
+#pragma unroll
+                for (int i = 0; i < NL; i++) {
+                    t0[i] = step != 1 ? r[i] : t0[i];
+                    t1[i] = step == 1 ? r[i] : t1[i];
+                }
+            }
 #pragma unroll
             for (int i = 0; i < NL; i++) {
-                t0[i] = step != 1 ? r[i] : t0[i];
-                t1[i] = step == 1 ? r[i] : t1[i];
+                u0[i] = u1[i];
+                u1[i] = u2[i];
+                u2[i] = u3[i];
+                u3[i] = t0[i];
             }
         }
         st<NL>(dst, o, t0);

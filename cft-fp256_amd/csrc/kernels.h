@@ -22,8 +22,8 @@ hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *
                         size_t n, uint32_t smask, uint32_t *flags, int grid, hipStream_t st);
 
 // One launch of the reductions' tree: for each of nseg segments of m
-// partials (segment s at src[s*m ..)), groups of four consecutive partials
-// become one node at dst[s*m2 + g], m2 = ceil(m/4).  leaf (the first launch
+// partials (segment s at src[s*m ..)), groups of sixteen consecutive partials
+// become one node at dst[s*m2 + g], m2 = ceil(m/16).  leaf (the first launch
 // only): 0 the partial itself, 1 its rounded product with srcb's, 2 its
 // magnitude, 3 its rounded square; kind: 0 ADD nodes, 1 maximum nodes.  Flags are ORed into
 // segflags[s] (per_seg) or segflags[0]; segspec, when given, gets bit 0 where
