@@ -567,12 +567,24 @@ __global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32
 
 #define WIDE_FENCE() __builtin_amdgcn_wave_barrier()
 
+// A step record, read through the address space it lives in (the launch
+// says which, uniformly): ds_read from LDS, global_load from HBM.  A generic
+// (flat) load would do for both, but the compiler can then only wait for it
+// with every counter drained, which defeats fetching the next step's records
+// while the current step computes.
+typedef unsigned int RecV __attribute__((ext_vector_type(4)));
+typedef const RecV __attribute__((address_space(3))) *LdsRec;
+typedef const RecV __attribute__((address_space(1))) *HbmRec;
+// (The typed base pointers are set once from the generic ones and indexed
+// as typed; a cast at each load would be folded back to a flat load.)
+
 // One instance of a wide schedule: its records, its register file in LDS and
 // its scratch slots in HBM.
 template <int NL>
 struct WideInst {
     using RT = typename Row<NL>::T;
-    const uint4 *recs;
+    LdsRec recs_lds;          // the records in LDS (wide_run<NL, true>)
+    HbmRec recs_hbm;          // or in HBM (wide_run<NL, false>)
     uint32_t nsteps, W, j, top_step, nvregs, seg, reach, n_in, n_out;
     const RT *konst;          // constant k at konst + k * NQ
     RT *regs;                 // register r at regs + r * rstride
@@ -581,12 +593,23 @@ struct WideInst {
     size_t pstride;
 };
 
+template <int NL, bool RECS_LDS>
+__device__ __forceinline__ uint4 wide_rec(const WideInst<NL> &I, uint32_t k)
+{
+    RecV v;
+    if constexpr (RECS_LDS)
+        v = I.recs_lds[k];
+    else
+        v = I.recs_hbm[k];
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Every segment of one instance, its W lanes together: `left` trips of the
 // segment loop as segments of `seg` trips and a last one of the remainder,
 // each a fresh run (registers +0, r0..r2 from the streams, scratch cleared
 // to reach, the state as the scratch-in block, the scratch-out block back
 // into the state).  Returns this lane's flags.
-template <int NL>
+template <int NL, bool RECS_LDS>
 __device__ __forceinline__ uint32_t wide_run(const WideInst<NL> &I, const bool valid,
                                              uint32_t left, uint32_t *st,
                                              const uint32_t *const src[3], const int phys[3],
@@ -631,9 +654,23 @@ __device__ __forceinline__ uint32_t wide_run(const WideInst<NL> &I, const bool v
         uint32_t body0 = 0, body1 = 0, body2 = 0, body3 = 0;
         int level = 0;
         uint32_t pc = 0;
+        // step nx's records are fetched while step pc computes; a jump
+        // (REPEAT skipped, ENDREP taken) refetches
+        uint32_t nx = 0xFFFFFFFFu;
+        uint4 n0 = make_uint4(0, 0, 0, 0), nr = n0;
         while (pc < I.nsteps) {
             pc = uni(pc);
-            const uint4 r0 = I.recs[(size_t)pc * W];
+            uint4 r0, rec;
+            if (pc == nx) {
+                r0 = n0;
+                rec = nr;
+            } else {
+                r0 = wide_rec<NL, RECS_LDS>(I, pc * W);
+                rec = wide_rec<NL, RECS_LDS>(I, pc * W + j);
+            }
+            nx = pc + 1 < I.nsteps ? pc + 1 : pc;
+            n0 = wide_rec<NL, RECS_LDS>(I, nx * W);
+            nr = wide_rec<NL, RECS_LDS>(I, nx * W + j);
             const uint32_t x0 = uni(r0.x);
             const bool en = active && park == 0;
             if (WR_KIND(x0) == WK_CTRL) {
@@ -683,7 +720,6 @@ __device__ __forceinline__ uint32_t wide_run(const WideInst<NL> &I, const bool v
                 }
                 continue;
             }
-            const uint4 rec = j == 0 ? r0 : I.recs[(size_t)pc * W + j];
             const uint32_t kind = WR_KIND(rec.x);
             if (en && kind != WK_NOP) {
                 if (kind == WK_ALU) {
@@ -796,7 +832,9 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
         __syncthreads();
     }
     WideInst<NL> I;
-    I.recs = Wd.recs_in_lds ? s_recs : Wd.recs;
+    I.recs_lds = (LdsRec)s_recs;
+    I.recs_hbm = (HbmRec)Wd.recs;
+    const bool recs_lds = Wd.recs_in_lds != 0;
     I.nsteps = Wd.nsteps;
     I.W = W;
     I.j = j;
@@ -828,7 +866,8 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
         const uint32_t id = valid ? B.order[idx] : 0u;
         const uint32_t left = valid ? B.steps[id] : 0u;
         uint32_t *st = B.state + (size_t)id * A.n_in * NL;
-        const uint32_t lfl = wide_run<NL>(I, valid, left, st, src, phys, id);
+        const uint32_t lfl = recs_lds ? wide_run<NL, true>(I, valid, left, st, src, phys, id)
+                                      : wide_run<NL, false>(I, valid, left, st, src, phys, id);
         const uint32_t inst = instance_flags(s_lflags, g, j, lfl);
         if (j == 0 && valid && B.lflags)
             B.lflags[id] = (uint8_t)inst;
@@ -883,7 +922,8 @@ __global__ __launch_bounds__(256) void k_seq_multi(SeqArgs A, SeqMulti M)
             staged = pi;
         }
         WideInst<NL> I;
-        I.recs = P->recs;
+        I.recs_lds = nullptr;
+        I.recs_hbm = (HbmRec)P->recs;
         I.nsteps = uni(P->nsteps);
         I.W = W;
         I.j = j;
@@ -902,7 +942,7 @@ __global__ __launch_bounds__(256) void k_seq_multi(SeqArgs A, SeqMulti M)
         const uint32_t pos = first + (valid ? g : 0u);
         uint32_t *st = valid ? reinterpret_cast<uint32_t *>(M.saddr[pos]) : nullptr;
         const uint32_t left = valid ? M.steps[pos] : 0u;
-        const uint32_t lfl = wide_run<NL>(I, valid, left, st, src, phys, pos);
+        const uint32_t lfl = wide_run<NL, false>(I, valid, left, st, src, phys, pos);
         const uint32_t inst = instance_flags(s_lflags, g, j, lfl);
         if (j == 0 && valid && M.lflags)
             M.lflags[pos] = (uint8_t)inst;
