@@ -650,6 +650,115 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     round_fixed<NL, WN, T>(sign, S, M, VEb + (M - T), rnd, out, flags);
 }
 
+// a + c, or a - c when neg_c: the value, bits and flags of fma_core's ADD /
+// SUB (a*1 + c, one rounding; host/src/softfloat.c:438-468 and the special
+// order of 324-428), in an NL-limb window instead of the FMA's 2NL.
+//
+// Method.  Each significand (implicit bit for a normal, raw for a
+// subnormal: no normalisation is needed to add) goes 3 bits up, so bits
+// 0..2 lie below both terms; the term with the smaller lsb exponent is
+// shifted right by the difference, every bit that falls off ORed into bit 0
+// with the bit already there.  P + 4 <= 32 NL for every format, so the sum
+// and its sign fit.  Unless the terms cancel deeply (lsb exponents within
+// one, opposite signs) or both are tiny, the sum's leading bit is at P+1,
+// P+2 or P+3: the guard is then at bit 1 or above, every rounding boundary a
+// multiple of 2, and the jammed sum lies in the same open interval between
+// consecutive boundaries as the exact one.  The deep cases shift nothing
+// off, so they are exact and take the general normalisation.
+template <int NL>
+__device__ __forceinline__ void add_core(const uint32_t (&xa)[NL], const uint32_t (&xc)[NL],
+                                         bool neg_c, int rnd, uint32_t (&out)[NL],
+                                         uint32_t &flags)
+{
+    using Fm = F<NL>;
+    constexpr int P = Fm::P;
+    static_assert(P + 4 <= 32 * NL, "the shifted sum must fit the window");
+    Unp<NL> ua, uc;
+    unpack_raw<NL>(xa, ua);
+    unpack_raw<NL>(xc, uc);
+    if (neg_c)
+        uc.sign ^= 1;
+    if (ua.nan || uc.nan || ua.inf || uc.inf || ua.zero) {
+        if (ua.nan || uc.nan) {
+            enc_qnan<NL>(out);
+            flags |= (ua.snan || uc.snan) ? FL_INV : 0u;
+        } else if (ua.inf) {
+            if (uc.inf && uc.sign != ua.sign) {
+                enc_qnan<NL>(out);
+                flags |= FL_INV;
+            } else {
+                enc_inf<NL>(out, ua.sign);
+            }
+        } else if (uc.inf) {
+            enc_inf<NL>(out, uc.sign);
+        } else if (uc.zero) {
+            enc_zero<NL>(out, uc.sign == ua.sign ? uc.sign : (rnd == RDN));
+        } else {
+            // 0 + c is c, bits and all (SUB's flipped sign)
+#pragma unroll
+            for (int i = 0; i < NL - 1; i++)
+                out[i] = xc[i];
+            out[NL - 1] = xc[NL - 1] ^ (neg_c ? 0x80000000u : 0u);
+        }
+        return;
+    }
+    // both finite, a nonzero; a zero c sits below a and adds nothing
+    const bool cbig = uc.e > ua.e;
+    const int d = cbig ? uc.e - ua.e : ua.e - uc.e;
+    const int eX = cbig ? uc.e : ua.e;
+    uint32_t X[NL], Y[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        const uint32_t a3 = alignbit(ua.m[i], i ? ua.m[i - 1] : 0u, 29);   // m << 3
+        const uint32_t c3 = alignbit(uc.m[i], i ? uc.m[i - 1] : 0u, 29);
+        X[i] = cbig ? c3 : a3;
+        Y[i] = cbig ? a3 : c3;
+    }
+    int sign = cbig ? uc.sign : ua.sign;
+    const bool subtract = ua.sign != uc.sign;
+    {
+        uint32_t jam = 0;
+        shr_jam<NL>(Y, (uint32_t)(d > 32 * NL ? 32 * NL : d), jam);
+        Y[0] |= jam ? 1u : 0u;
+    }
+    uint32_t S[NL];
+    {
+        const uint32_t m = subtract ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+            Y[i] ^= m;
+        add_limbs<NL>(S, X, Y, subtract ? 1u : 0u);
+    }
+    constexpr int T1 = P + 1, T2 = P + 2, T3 = P + 3;
+    const bool b3 = (S[T3 / 32] >> (T3 % 32)) & 1u;
+    const bool b2 = (S[T2 / 32] >> (T2 % 32)) & 1u;
+    const bool b1 = (S[T1 / 32] >> (T1 % 32)) & 1u;
+    const bool neg = (S[NL - 1] >> 31) & 1u;     // only with equal lsb exponents
+    if (neg || (!b3 && !b2 && !b1)) {
+        // rare: deep cancellation or a tiny sum, exact either way
+        if (neg) {
+            neg_limbs<NL>(S);
+            sign ^= 1;
+        }
+        if (is_zero<NL>(S)) {                    // exact cancellation
+            enc_zero<NL>(out, rnd == RDN);
+            return;
+        }
+        round_window<NL, NL>(sign, S, eX - 3, rnd, out, flags);
+        return;
+    }
+    const int lead = b3 ? T3 : (b2 ? T2 : T1);
+    const uint32_t off = (uint32_t)(lead - P + 1);          // 2, 3 or 4
+    uint32_t K[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        K[i] = alignbit(i + 1 < NL ? S[i + 1] : 0u, S[i], off);
+    K[NL - 1] &= (2u << Fm::TOPF) - 1u;
+    const bool g = (S[0] >> (off - 1)) & 1u;
+    const bool st = (S[0] & ((1u << (off - 1)) - 1u)) != 0;
+    round_pack_norm<NL>(sign, K, g, st, eX - 3 + lead, rnd, out, flags);
+}
+
 template <int NL, int OP>
 __device__ __forceinline__ void fma_op(const uint32_t (&xa)[NL], const uint32_t (&xb)[NL],
                                        const uint32_t (&xc)[NL], int rnd, uint32_t (&out)[NL],

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
                     y[i] = step == 0 ? v1[i] : step == 1 ? v3[i] : t1[i];
                 }
                 if constexpr (KIND == NODE_ADD)
-                    fma_core<NL>(OP_ADD, x, x, y, rnd, r, fl);    // x * 1 + y
+                    add_core<NL>(x, y, false, rnd, r, fl);        // x + y
                 else
                     misc_op<NL>(8, x, y, y, r, fl);                // 754-2019 maximum
 #pragma unroll

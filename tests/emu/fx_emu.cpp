@@ -27,6 +27,8 @@ static void one(int op, int rnd, const uint32_t *a, const uint32_t *b, const uin
     case 3: fma_op<NL, OP_MUL>(xa, xb, xc, rnd, o, fl); break;
     case 256: div_op<NL>(xa, xb, rnd, o, fl); break;
     case 257: sqrt_op<NL>(xa, rnd, o, fl); break;
+    case 258: add_core<NL>(xa, xc, false, rnd, o, fl); break;   // the two-term ADD path
+    case 259: add_core<NL>(xa, xc, true, rnd, o, fl); break;    // and SUB
     default: misc_op<NL>(op, xa, xb, xc, o, fl); break;
     }
     memcpy(out, o, sizeof o);
@@ -60,4 +62,35 @@ extern "C" int emu_one(int fmt, int op, int rnd, const void *a, const void *b, c
                        void *d, uint32_t *flags)
 {
     return emu_run(fmt, op, rnd, a, b, c, d, 1, flags);
+}
+
+// The two-term add path against the FMA path's ADD / SUB, element by
+// element, bits and flags: returns the number of differing elements and
+// the first one's index in *first.
+extern "C" size_t emu_cmp_add(int fmt, int rnd, int sub, const void *a, const void *c, size_t n,
+                              size_t *first)
+{
+    const int nl = 1 << fmt;
+    size_t bad = 0;
+    *first = (size_t)-1;
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t *pa = (const uint32_t *)a + i * nl, *pc = (const uint32_t *)c + i * nl;
+        uint32_t d0[8], d1[8], f0 = 0, f1 = 0;
+        switch (fmt) {
+        case 0: one<1>(sub ? 2 : 1, rnd, pa, nullptr, pc, d0, &f0);
+                one<1>(sub ? 259 : 258, rnd, pa, nullptr, pc, d1, &f1); break;
+        case 1: one<2>(sub ? 2 : 1, rnd, pa, nullptr, pc, d0, &f0);
+                one<2>(sub ? 259 : 258, rnd, pa, nullptr, pc, d1, &f1); break;
+        case 2: one<4>(sub ? 2 : 1, rnd, pa, nullptr, pc, d0, &f0);
+                one<4>(sub ? 259 : 258, rnd, pa, nullptr, pc, d1, &f1); break;
+        default: one<8>(sub ? 2 : 1, rnd, pa, nullptr, pc, d0, &f0);
+                 one<8>(sub ? 259 : 258, rnd, pa, nullptr, pc, d1, &f1); break;
+        }
+        if (memcmp(d0, d1, 4 * (size_t)nl) != 0 || f0 != f1) {
+            if (!bad)
+                *first = i;
+            bad++;
+        }
+    }
+    return bad;
 }

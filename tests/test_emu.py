@@ -56,3 +56,76 @@ def test_emu_workload_configs(emu, oracle):
             want, wf = oracle_elem(oracle, fmt, 0, rnd, a, b, c)
             got, gf = emu_elem(emu, fmt, 0, rnd, a, b, c)
             assert np.array_equal(got, want) and gf == wf, (cfg, rnd)
+
+
+def _add_cases(rng, fmt, n):
+    """Operand pairs for a two-term add: random encodings, and pairs built to
+    meet every branch - exponents within a few bits (deep cancellation, the
+    leading bit at each of its three places), differences around the jam
+    boundaries (1..P+6 bits), subnormal pairs, exact cancellations, zeros,
+    infinities and NaNs against anything."""
+    ew, mw, w = {0: (8, 23, 32), 1: (11, 52, 64), 2: (15, 112, 128), 3: (19, 236, 256)}[fmt]
+    esz, p = w // 8, mw + 1
+    emax = (1 << ew) - 1
+    enc = lambda s, e, m: ((s << (w - 1)) | (e << mw) | m).to_bytes(esz, "little")
+    A, C = [], []
+    for i in range(n):
+        kind = i % 8
+        s1, s2 = int(rng.integers(0, 2)), int(rng.integers(0, 2))
+        m1 = int.from_bytes(rng.bytes(esz), "little") % (1 << mw)
+        m2 = int.from_bytes(rng.bytes(esz), "little") % (1 << mw)
+        e1 = int(rng.integers(1, emax))
+        if kind == 0:                                   # anything
+            a, c = _rand_enc(rng, fmt, 1).tobytes(), _rand_enc(rng, fmt, 1).tobytes()
+        elif kind == 1:                                 # close exponents, opposite signs
+            e2 = min(max(e1 + int(rng.integers(-2, 3)), 1), emax - 1)
+            if rng.random() < 0.5:                      # share most high bits
+                m2 = (m1 ^ int(rng.integers(0, 1 << min(mw, 40)))) % (1 << mw)
+            a, c = enc(s1, e1, m1), enc(1 - s1, e2, m2)
+        elif kind == 2:                                 # jam boundaries
+            dd = int(rng.integers(0, p + 7))
+            e2 = e1 - dd if e1 - dd >= 1 else e1 + dd
+            e2 = min(max(e2, 1), emax - 1)
+            a, c = enc(s1, e1, m1), enc(s2, e2, m2)
+        elif kind == 3:                                 # subnormal with subnormal or tiny
+            a = enc(s1, 0, m1 or 1)
+            c = enc(s2, int(rng.integers(0, 3)), m2 or 1)
+        elif kind == 4:                                 # exact cancellation
+            a = enc(s1, e1, m1)
+            c = enc(1 - s1, e1, m1)
+        elif kind == 5:                                 # near the top of the range
+            a, c = enc(s1, emax - 1 - int(rng.integers(0, 3)), m1), enc(s1, emax - 1, m2)
+        elif kind == 6:                                 # zeros and specials
+            sp = [enc(0, 0, 0), enc(1, 0, 0), enc(0, emax, 0), enc(1, emax, 0),
+                  enc(0, emax, 1 << (mw - 1)), enc(0, emax, 1), enc(s1, e1, m1)]
+            a, c = sp[int(rng.integers(0, 7))], sp[int(rng.integers(0, 7))]
+        else:                                           # one subnormal, one normal
+            a, c = enc(s1, 0, m1 or 1), enc(s2, int(rng.integers(1, p + 3)), m2)
+        if rng.random() < 0.5:
+            a, c = c, a
+        A.append(a)
+        C.append(c)
+    return (np.frombuffer(b"".join(A), np.uint8).copy(),
+            np.frombuffer(b"".join(C), np.uint8).copy())
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_two_term_add_is_the_fma_paths_add(emu, oracle, fmt):
+    """add_core (the reductions' node) against fma_core's ADD and SUB element
+    by element, bits and flags, and against the oracle's."""
+    emu.emu_cmp_add.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_size_t,
+                                ctypes.POINTER(ctypes.c_size_t)]
+    emu.emu_cmp_add.restype = ctypes.c_size_t
+    rng = np.random.default_rng(300 + fmt)
+    n = 24000
+    a, c = _add_cases(rng, fmt, n)
+    for rnd in range(5):
+        for sub in (0, 1):
+            first = ctypes.c_size_t()
+            bad = emu.emu_cmp_add(fmt, rnd, sub, a.ctypes.data, c.ctypes.data, n,
+                                  ctypes.byref(first))
+            assert bad == 0, (fmt, rnd, sub, bad, first.value)
+            want, wf = oracle_elem(oracle, fmt, 2 if sub else 1, rnd, a, None, c)
+            got, gf = emu_elem(emu, fmt, 259 if sub else 258, rnd, a, None, c)
+            assert np.array_equal(got, want) and gf == wf, (fmt, rnd, sub)
