@@ -98,7 +98,10 @@ __global__ __launch_bounds__(256) void k_arith(const uint32_t *__restrict__ a,
         else
 #pragma unroll
             for (int k = 0; k < NL; k++) xc[k] = 0;
-        fma_op<NL, OP>(xa, xb, xc, rnd, o, fl);
+        if constexpr (OP == OP_ADD || OP == OP_SUB)
+            add_core<NL>(xa, xc, OP == OP_SUB, rnd, o, fl);   // a * 1 + c, one rounding
+        else
+            fma_op<NL, OP>(xa, xb, xc, rnd, o, fl);
         store_el<NL>(d, i, o);
     }
     flush_flags(fl, flags_out);
