@@ -732,7 +732,12 @@ __device__ __forceinline__ uint32_t wide_run(const WideInst<NL> &I, const bool v
                     if (WR_KC(rec.x)) row_get<NL>(I.konst + ic * NQ, 1, xc);
                     else row_get<NL>(I.regs + ic * rstride, 1, xc);
                     const uint32_t op = WR_OP(rec.x);
-                    if (op <= 3) {
+                    const bool addsub = op == OP_ADD || op == OP_SUB;
+                    if (__builtin_amdgcn_ballot_w64(!addsub) == 0) {
+                        // every ALU lane of the bundle adds or subtracts (a ring's
+                        // components side by side): the two-term path, same bits
+                        add_core<NL>(xa, xc, op == OP_SUB, (int)WR_RND(rec.x), o, f);
+                    } else if (op <= 3) {
                         // the lanes of a bundle mix FMA, ADD, SUB and MUL: steer each
                         // to the FMA it is defined as (host/src/softfloat.c sf_steer:
                         // ADD a*1+c, SUB a*1+(-c), MUL a*b+(+-0, the product's sign))
