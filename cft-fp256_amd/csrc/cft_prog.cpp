@@ -1000,7 +1000,7 @@ cft_status check_scratch(const cft_program *p, const cft_run_args *A, const char
 }
 
 // seq_check_round2 (program.c:2050-2250): index tables, lane mask, lane flags.
-cft_status check_round2(const cft_program *p, const cft_run_args *A, const char *who)
+cft_status check_indexed_args(const cft_program *p, const cft_run_args *A, const char *who)
 {
     const uint32_t *idx[3] = {A->idx_a, A->idx_b, A->idx_c};
     const size_t src[3] = {A->idx_a_src, A->idx_b_src, A->idx_c_src};
@@ -1305,7 +1305,7 @@ cft_status seq_check_run(cft_program *prog, const cft_run_args *args, const char
     if (st == CFT_OK)
         st = check_scratch(prog, args, who);
     if (st == CFT_OK)
-        st = check_round2(prog, args, who);
+        st = check_indexed_args(prog, args, who);
     return st;
 }
 

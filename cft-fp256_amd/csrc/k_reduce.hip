@@ -144,19 +144,19 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
         }
         const uint64_t left = m - 16 * g;
         const uint32_t cnt16 = left < 16 ? (uint32_t)left : 16u;
-        // rounds 0..3 reduce sub-group r (partials 4r .. 4r+3 of the group)
-        // to one sub-tree, rotated into u3; round 4 reduces the sub-trees
+        // turns 0..3 reduce sub-group t (partials 4t .. 4t+3 of the group)
+        // to one sub-tree, rotated into u3; turn 4 reduces the sub-trees
         uint32_t u0[NL] = {}, u1[NL] = {}, u2[NL] = {}, u3[NL] = {};
         uint32_t t0[NL], t1[NL];
 #pragma unroll 1
-        for (uint32_t rd = 0; rd < 5; rd++) {
+        for (uint32_t turn = 0; turn < 5; turn++) {
             uint32_t cnt;
             uint32_t v0[NL] = {}, v1[NL] = {}, v2[NL] = {}, v3[NL] = {};
-            if (rd < 4) {
+            if (turn < 4) {
                 // the sub-group's partials, every load issued before the
                 // first is used; absent ones stay +0 and are never combined
-                cnt = cnt16 > 4 * rd ? min(cnt16 - 4 * rd, 4u) : 0u;
-                const uint64_t b4 = base + 4 * rd;
+                cnt = cnt16 > 4 * turn ? min(cnt16 - 4 * turn, 4u) : 0u;
+                const uint64_t b4 = base + 4 * turn;
                 if (cnt > 0)
                     ld<NL>(src, b4, v0);
                 if (cnt > 1)
