@@ -229,3 +229,28 @@ def test_power_of_two_segments_compose_to_the_whole(device, fmt):
         top, ft = device.reduce(24, fmt, rnd, parts, n=n // seg)
         assert whole.tobytes() == top.tobytes(), (op, rnd)
         assert fw == fp | ft, (op, rnd)
+
+
+def test_results_over_their_own_operands(device, oc):
+    # d aliasing a: the segments' results written over the array's own head,
+    # from host memory and inside one resident buffer
+    fmt, n, seg = 2, 3000, 30
+    esz = ESZ[fmt]
+    rng = np.random.default_rng(23)
+    a = red_values(rng, fmt, n, special=0.01)
+    for op in (24, 28, 31):
+        want = run_oc(oc, fmt, op, 1, a, None, n, seg)
+        x = a.copy()
+        _, fl = device.reduce(op, fmt, 1, x, out=x[: (n // seg) * esz], n=n, seg=seg)
+        assert x[: (n // seg) * esz].tobytes() == want[0].tobytes() and fl == want[1], op
+        buf = Buffer(device, n * esz)
+        try:
+            buf.host[:] = a
+            _, fl = device.reduce(op, fmt, 1, buf, out=buf.host[: (n // seg) * esz], n=n,
+                                  seg=seg)
+            buf.from_device()
+            assert buf.host[: (n // seg) * esz].tobytes() == want[0].tobytes(), op
+            assert buf.host[(n // seg) * esz:].tobytes() == a[(n // seg) * esz:].tobytes()
+            assert fl == want[1], op
+        finally:
+            buf.free()
