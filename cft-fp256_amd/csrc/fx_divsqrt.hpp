@@ -42,7 +42,16 @@ __device__ __forceinline__ void long_div(uint32_t (&U)[2 * NL + 1], const uint32
             const double num = (double)U[j + NL] * 18446744073709551616.0 +
                                (double)U[j + NL - 1] * 4294967296.0 +
                                (double)(j + NL - 2 >= 0 ? U[(j + NL - 2) < 0 ? 0 : j + NL - 2] : 0u);
-            double qd = num * rinv + 1.0;        // at or above the digit
+            // At or above the digit: num, dtop, rinv and the product each
+            // carry one rounding, so num * rinv is within 5 * 2^-53
+            // relative, 2^-18.7 absolute (the quotient is below 2^32), of
+            // U3/D2 (the top three remainder limbs over the top two divisor
+            // limbs), and the true digit is at most floor((U3 + 1)/D2),
+            // 2^-63 above U3/D2.  So + 2^-17 never undershoots, and
+            // overshoots only when U3/D2 lies within 2^-17 below an integer
+            // (or by Knuth's truncated-divisor bound): the add-back below
+            // almost never runs.
+            double qd = num * rinv + 0x1p-17;
             qd = qd > 4294967295.0 ? 4294967295.0 : (qd < 0.0 ? 0.0 : qd);
             qhat = (uint32_t)qd;
         }
