@@ -216,7 +216,13 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         for (int i = 0; i <= NL; i++)
             if (i <= k && i >= k - 2)
                 Vd += ldexp((double)V[i], 32 * (i + 1));
-        double dd = Rd / Vd + 2.0;               // at or above the digit
+        // At or above the digit: R's dropped limbs move R / (V B) by under
+        // 2^-63 (V B >= 2^64), V's only raise the estimate, and the sums and
+        // the quotient (below 2^33) carry 6.5 * 2^-53 relative, 2^-17.3
+        // absolute, of rounding; the digit itself is at most R / (V B).  So
+        // + 2^-15 never undershoots, and the steps down below run for the
+        // d^2 / (V B) < 1 the estimate ignores.
+        double dd = Rd / Vd + 0x1p-15;
         dd = dd > 4294967295.0 ? 4294967295.0 : (dd < 0.0 ? 0.0 : dd);
         uint32_t d = (uint32_t)dd;
         // T = (V * B + d) * d, k + 3 limbs
