@@ -55,34 +55,43 @@ __device__ __forceinline__ void long_div(uint32_t (&U)[2 * NL + 1], const uint32
             qd = qd > 4294967295.0 ? 4294967295.0 : (qd < 0.0 ? 0.0 : qd);
             qhat = (uint32_t)qd;
         }
-        // U[j .. j+NL] -= qhat * D
-        uint32_t carry = 0, br = 0;
+        // U[j .. j+NL] -= qhat * D: the NL partial products independently,
+        // their limbs assembled by one carry chain and subtracted by one
+        // borrow chain
+        uint32_t pl[NL], ph[NL];
 #pragma unroll
         for (int i = 0; i < NL; i++) {
-            const uint64_t p = (uint64_t)qhat * D[i] + carry;
-            carry = (uint32_t)(p >> 32);
-            const uint64_t t = (uint64_t)U[j + i] - (uint32_t)p - br;
-            U[j + i] = (uint32_t)t;
-            br = (uint32_t)(t >> 63);
+            const uint64_t p = (uint64_t)qhat * D[i];
+            pl[i] = (uint32_t)p;
+            ph[i] = (uint32_t)(p >> 32);
+        }
+        uint32_t c = 0, br = 0;
+#pragma unroll
+        for (int i = 0; i < NL; i++) {
+            uint32_t co, bo;
+            const uint32_t pi = addc(pl[i], i ? ph[i - 1] : 0u, c, &co);
+            c = co;
+            U[j + i] = subc(U[j + i], pi, br, &bo);
+            br = bo;
         }
         {
-            const uint64_t t = (uint64_t)U[j + NL] - carry - br;
-            U[j + NL] = (uint32_t)t;
-            br = (uint32_t)(t >> 63);
+            uint32_t bo;                         // qhat * D < 2^(32(NL+1)): no carry past
+            U[j + NL] = subc(U[j + NL], ph[NL - 1] + c, br, &bo);
+            br = bo;
         }
         // negative: add D back until it is not (the exact test decides)
         bool neg = br != 0;
         while (neg) {
-            uint64_t c = 0;
+            uint32_t c = 0;
 #pragma unroll
             for (int i = 0; i < NL; i++) {
-                c += (uint64_t)U[j + i] + D[i];
-                U[j + i] = (uint32_t)c;
-                c >>= 32;
+                uint32_t co;
+                U[j + i] = addc(U[j + i], D[i], c, &co);
+                c = co;
             }
-            c += U[j + NL];
-            U[j + NL] = (uint32_t)c;
-            neg = (c >> 32) == 0;                // no carry out: still negative
+            uint32_t co;
+            U[j + NL] = addc(U[j + NL], 0u, c, &co);
+            neg = co == 0;                       // no carry out: still negative
             qhat--;
         }
         Q[j] = qhat;

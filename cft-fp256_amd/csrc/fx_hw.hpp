@@ -36,4 +36,10 @@ __device__ __forceinline__ uint32_t addc(uint32_t x, uint32_t y, uint32_t cin, u
     return __builtin_addc(x, y, cin, cout);
 }
 
+// x - y - bin (bin 0 or 1), borrow out: one v_sub_co / v_subb in a chain.
+__device__ __forceinline__ uint32_t subc(uint32_t x, uint32_t y, uint32_t bin, uint32_t *bout)
+{
+    return __builtin_subc(x, y, bin, bout);
+}
+
 } // namespace fx

@@ -34,4 +34,11 @@ inline uint32_t addc(uint32_t x, uint32_t y, uint32_t cin, uint32_t *cout)
     return (uint32_t)s;
 }
 
+inline uint32_t subc(uint32_t x, uint32_t y, uint32_t bin, uint32_t *bout)
+{
+    const uint64_t t = (uint64_t)x - y - bin;
+    *bout = (uint32_t)(t >> 63);
+    return (uint32_t)t;
+}
+
 } // namespace fx
