@@ -9,6 +9,7 @@
 // allocation refused before anything ran is CFT_ERR_OUT_OF_MEMORY and the
 // handle survives.
 #include "cft_internal.hpp"
+#include "kernels.h"
 
 #include <limits.h>
 #include <stdio.h>
@@ -470,9 +471,16 @@ static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft
         return CFT_ERR_NO_DEVICE;
     }
     if (hipMalloc((void **)&D->dwords, 256) != hipSuccess ||
-        hipHostMalloc((void **)&D->hwords, 256, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&D->hwords, 256, hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&D->hwords_dev, D->hwords, 0) != hipSuccess ||
+        hipMalloc((void **)&D->dslots, CFTK_FLAG_SLOTS * CFTK_SLOT_WORDS * 4) != hipSuccess ||
+        hipMemsetAsync(D->dslots, 0, CFTK_FLAG_SLOTS * CFTK_SLOT_WORDS * 4, D->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(D->stream) != hipSuccess) {
         (void)hipGetLastError();
         if (D->dwords) (void)hipFree(D->dwords);
+        if (D->hwords) (void)hipHostFree(D->hwords);
+        if (D->dslots) (void)hipFree(D->dslots);
         (void)hipStreamDestroy(D->stream);
         delete D;
         cft_set_error("cft_open: %s: the handle's flag words could not be allocated",
@@ -556,6 +564,7 @@ extern "C" CFT_API void cft_close(cft_device *dev)
         (void)hipFree(p);
     if (dev->dwords) (void)hipFree(dev->dwords);
     if (dev->hwords) (void)hipHostFree(dev->hwords);
+    if (dev->dslots) (void)hipFree(dev->dslots);
     if (dev->ev0) (void)hipEventDestroy(dev->ev0);
     if (dev->ev1) (void)hipEventDestroy(dev->ev1);
     (void)hipStreamDestroy(dev->stream);

@@ -14,6 +14,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 namespace {
@@ -58,18 +59,22 @@ cft_status execute(const char *who, cft_device *dev, int which, int op, int fmt,
         (st = bind_in(dev, ROLE_C, c, nc * esz, &dc, who)) != CFT_OK ||
         (st = bind_out(dev, ROLE_D, d, n * esz, &dd, &dbuf, who)) != CFT_OK)
         return st;
-    hipError_t e = cftk_fill32(dev->dwords, 0u, 1, dev->stream);
+    // The kernel ORs its flags into the handle's flag slots (zero between
+    // calls); the gather launch after it ORs the slots into the pinned host
+    // word and zeroes them again: no clearing launch before, no copy after.
+    const int grid = grid_for(dev, n);
+    const int nslots = std::min(grid, (int)CFTK_FLAG_SLOTS);
+    const uint32_t slot_mask = (uint32_t)(nslots > 1 ? CFTK_FLAG_SLOTS - 1 : 0);
+    hipError_t e = timer_begin(dev);
     if (e == hipSuccess)
-        e = timer_begin(dev);
-    if (e == hipSuccess) {
-        const int grid = grid_for(dev, n);
-        e = which < 0 ? cftk_elem(fmt, op, rnd, da, db, dc, dd, n, smask, dev->dwords, grid,
-                                  dev->stream)
-                      : cftk_divsqrt(fmt, which, rnd, da, db, dd, n, smask, dev->dwords, grid,
-                                     dev->stream);
-    }
+        e = which < 0 ? cftk_elem(fmt, op, rnd, da, db, dc, dd, n, smask, dev->dslots, slot_mask,
+                                  grid, dev->stream)
+                      : cftk_divsqrt(fmt, which, rnd, da, db, dd, n, smask, dev->dslots,
+                                     slot_mask, grid, dev->stream);
     if (e == hipSuccess)
         e = timer_end(dev);
+    if (e == hipSuccess)
+        e = cftk_flags_gather(dev->dslots, nslots, dev->hwords_dev, dev->stream);
     if (e != hipSuccess)
         return dev_fail(dev, e, who, "launching the kernel");
     if (!dbuf) {
@@ -77,15 +82,13 @@ cft_status execute(const char *who, cft_device *dev, int which, int op, int fmt,
         if (e != hipSuccess)
             return dev_fail(dev, e, who, "copying the result back");
     }
-    e = hipMemcpyAsync(dev->hwords, dev->dwords, 4, hipMemcpyDeviceToHost, dev->stream);
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(dev->stream);
+    e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess)
         return dev_fail(dev, e, who, "running the kernel");
     timer_collect(dev);
     if (dbuf)
         dbuf->host_valid = false;               // the device copy is the truth now
-    cft_emit_flags(dev, dev->hwords[0], flags_out);
+    cft_emit_flags(dev, ((volatile uint32_t *)dev->hwords)[0], flags_out);
     return CFT_OK;
 }
 

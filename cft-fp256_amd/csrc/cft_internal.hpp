@@ -67,6 +67,9 @@ struct cft_device {
     size_t lds_max = 65536;        // LDS bytes a workgroup may take
     uint32_t *dwords = nullptr;    // device: [0] flags, [1] status
     uint32_t *hwords = nullptr;    // pinned mirror of dwords
+    uint32_t *hwords_dev = nullptr;  // hwords' device address
+    // the elementwise calls' flag slots (kernels.h), zero between calls
+    uint32_t *dslots = nullptr;
     std::vector<cft_buffer *> bufs;
     // cftx_kernel_timer: events around each compute kernel on `stream`
     bool timing = false;

@@ -244,10 +244,10 @@ hipError_t launch_elem(Worker &w)
         const int grid = grid_for(ex, J.n);
         if (J.kind == CFTX_JOB_ELEM)
             e = cftk_elem(J.fmt, J.op, J.rnd, j->da, j->db, j->dc, j->dd, J.n, 0u, w.dflags + k,
-                          grid, ex->stream);
+                          0u, grid, ex->stream);
         else
             e = cftk_divsqrt(J.fmt, J.kind == CFTX_JOB_DIV ? 0 : 1, J.rnd, j->da, j->db, j->dd,
-                             J.n, 0u, w.dflags + k, grid, ex->stream);
+                             J.n, 0u, w.dflags + k, 0u, grid, ex->stream);
     }
     if (e == hipSuccess)
         e = hipMemcpyAsync(w.hflags, w.dflags, B.jobs.size() * 4, hipMemcpyDeviceToHost,

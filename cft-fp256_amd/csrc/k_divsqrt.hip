@@ -21,7 +21,8 @@ template <int NL, int WHICH>
 __global__ __launch_bounds__(256) void k_divsqrt(const uint32_t *__restrict__ a,
                                                  const uint32_t *__restrict__ b,
                                                  uint32_t *__restrict__ d, size_t n, int rnd,
-                                                 uint32_t *__restrict__ flags_out)
+                                                 uint32_t *__restrict__ flags_out,
+                                                 uint32_t slot_mask)
 {
     uint32_t fl = 0;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -38,8 +39,9 @@ __global__ __launch_bounds__(256) void k_divsqrt(const uint32_t *__restrict__ a,
         for (int k = 0; k < NL; k++)
             d[i * NL + k] = o[k];
     }
-    // the wavefront's OR of its lanes' flags, one atomic for bits not yet set
-    // (no LDS: see flush_flags in k_elem.hip)
+    // the wavefront's OR of its lanes' flags, one atomic into its flag slot
+    // for bits not yet set (no LDS: see flush_flags in k_elem.hip)
+    flags_out += (blockIdx.x & slot_mask) * CFTK_SLOT_WORDS;
     uint32_t w = 0;
 #pragma unroll
     for (int b = 0; b < 5; b++)
@@ -56,30 +58,30 @@ __global__ __launch_bounds__(256) void k_divsqrt(const uint32_t *__restrict__ a,
 
 template <int NL>
 hipError_t launch(int which, int rnd, const void *a, const void *b, void *d, size_t n,
-                  uint32_t *flags, int grid, hipStream_t st)
+                  uint32_t *flags, uint32_t sm, int grid, hipStream_t st)
 {
     if (which == 0)
         hipLaunchKernelGGL((k_divsqrt<NL, 0>), dim3(grid), dim3(256), 0, st,
                            (const uint32_t *)a, (const uint32_t *)b, (uint32_t *)d, n, rnd,
-                           flags);
+                           flags, sm);
     else
         hipLaunchKernelGGL((k_divsqrt<NL, 1>), dim3(grid), dim3(256), 0, st,
                            (const uint32_t *)a, (const uint32_t *)b, (uint32_t *)d, n, rnd,
-                           flags);
+                           flags, sm);
     return hipGetLastError();
 }
 
 } // namespace
 
 extern "C" hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *b,
-                                   void *d, size_t n, uint32_t smask, uint32_t *flags, int grid,
-                                   hipStream_t st)
+                                   void *d, size_t n, uint32_t smask, uint32_t *flags,
+                                   uint32_t slot_mask, int grid, hipStream_t st)
 {
     (void)smask;
     switch (fmt) {
-    case 0: return launch<1>(which, rnd, a, b, d, n, flags, grid, st);
-    case 1: return launch<2>(which, rnd, a, b, d, n, flags, grid, st);
-    case 2: return launch<4>(which, rnd, a, b, d, n, flags, grid, st);
-    default: return launch<8>(which, rnd, a, b, d, n, flags, grid, st);
+    case 0: return launch<1>(which, rnd, a, b, d, n, flags, slot_mask, grid, st);
+    case 1: return launch<2>(which, rnd, a, b, d, n, flags, slot_mask, grid, st);
+    case 2: return launch<4>(which, rnd, a, b, d, n, flags, slot_mask, grid, st);
+    default: return launch<8>(which, rnd, a, b, d, n, flags, slot_mask, grid, st);
     }
 }

@@ -6,8 +6,8 @@
 //
 // Replaces the reference's k_run / k_run256 / k_run128 (host/src/
 // backend_hip.cpp:484-665).  Launch shape: a grid-stride loop over a fixed
-// grid sized from the CU count, so the per-call flag word is ORed from one
-// atomic per workgroup rather than one per element or wave.
+// grid sized from the CU count; each wavefront ORs its flags into one of the
+// call's flag slots (kernels.h) with at most one atomic.
 #include "fx_arith.hpp"
 #include "fx_misc.hpp"
 #include "kernels.h"
@@ -81,7 +81,8 @@ __global__ __launch_bounds__(256) void k_arith(const uint32_t *__restrict__ a,
                                                const uint32_t *__restrict__ c,
                                                uint32_t *__restrict__ d, size_t n,
                                                uint32_t smask, int rnd,
-                                               uint32_t *__restrict__ flags_out)
+                                               uint32_t *__restrict__ flags_out,
+                                               uint32_t slot_mask)
 {
     uint32_t fl = 0;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(256) void k_arith(const uint32_t *__restrict__ a,
             fma_op<NL, OP>(xa, xb, xc, rnd, o, fl);
         store_el<NL>(d, i, o);
     }
-    flush_flags(fl, flags_out);
+    flush_flags(fl, flags_out + (blockIdx.x & slot_mask) * CFTK_SLOT_WORDS);
 }
 
 // Every other elementwise opcode (sign, min/max, compare, select, integer,
@@ -116,7 +117,8 @@ __global__ __launch_bounds__(256) void k_misc(const uint32_t *__restrict__ a,
                                               const uint32_t *__restrict__ c,
                                               uint32_t *__restrict__ d, size_t n,
                                               uint32_t smask, int op,
-                                              uint32_t *__restrict__ flags_out)
+                                              uint32_t *__restrict__ flags_out,
+                                              uint32_t slot_mask)
 {
     uint32_t fl = 0;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(256) void k_misc(const uint32_t *__restrict__ a,
         misc_op<NL>(op, xa, xb, xc, o, fl);
         store_el<NL>(d, i, o);
     }
-    flush_flags(fl, flags_out);
+    flush_flags(fl, flags_out + (blockIdx.x & slot_mask) * CFTK_SLOT_WORDS);
 }
 
 __global__ void k_fill32(uint32_t *p, uint32_t v, size_t n)
@@ -140,9 +142,29 @@ __global__ void k_fill32(uint32_t *p, uint32_t v, size_t n)
         p[i] = v;
 }
 
+// One wavefront: lane s takes slot s's word and zeroes it, the wave ORs the
+// words, lane 0 stores the OR to host memory.  The store is system scope,
+// so it is in host memory when the stream's completion is seen.
+__global__ __launch_bounds__(64) void k_flags_gather(uint32_t *slots, int nslots,
+                                                     uint32_t *host_word)
+{
+    const int s = (int)threadIdx.x;
+    uint32_t v = 0;
+    if (s < nslots) {
+        v = slots[s * CFTK_SLOT_WORDS];
+        slots[s * CFTK_SLOT_WORDS] = 0u;
+    }
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1)
+        v |= __shfl_xor(v, k, 64);
+    if (s == 0)
+        __hip_atomic_store(host_word, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <int NL>
 hipError_t launch_arith(int op, const void *a, const void *b, const void *c, void *d, size_t n,
-                        uint32_t smask, int rnd, uint32_t *flags, int grid, hipStream_t st)
+                        uint32_t smask, int rnd, uint32_t *flags, uint32_t sm, int grid,
+                        hipStream_t st)
 {
     const uint32_t *pa = (const uint32_t *)a, *pb = (const uint32_t *)b,
                    *pc = (const uint32_t *)c;
@@ -150,19 +172,19 @@ hipError_t launch_arith(int op, const void *a, const void *b, const void *c, voi
     switch (op) {
     case OP_FMA:
         hipLaunchKernelGGL((k_arith<NL, OP_FMA>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
-                           smask, rnd, flags);
+                           smask, rnd, flags, sm);
         break;
     case OP_ADD:
         hipLaunchKernelGGL((k_arith<NL, OP_ADD>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
-                           smask, rnd, flags);
+                           smask, rnd, flags, sm);
         break;
     case OP_SUB:
         hipLaunchKernelGGL((k_arith<NL, OP_SUB>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
-                           smask, rnd, flags);
+                           smask, rnd, flags, sm);
         break;
     default:
         hipLaunchKernelGGL((k_arith<NL, OP_MUL>), dim3(grid), dim3(256), 0, st, pa, pb, pc, pd, n,
-                           smask, rnd, flags);
+                           smask, rnd, flags, sm);
         break;
     }
     return hipGetLastError();
@@ -170,11 +192,11 @@ hipError_t launch_arith(int op, const void *a, const void *b, const void *c, voi
 
 template <int NL>
 hipError_t launch_misc(int op, const void *a, const void *b, const void *c, void *d, size_t n,
-                       uint32_t smask, uint32_t *flags, int grid, hipStream_t st)
+                       uint32_t smask, uint32_t *flags, uint32_t sm, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL((k_misc<NL>), dim3(grid), dim3(256), 0, st, (const uint32_t *)a,
                        (const uint32_t *)b, (const uint32_t *)c, (uint32_t *)d, n, smask, op,
-                       flags);
+                       flags, sm);
     return hipGetLastError();
 }
 
@@ -182,18 +204,19 @@ hipError_t launch_misc(int op, const void *a, const void *b, const void *c, void
 
 extern "C" hipError_t cftk_elem(int fmt, int op, int rnd, const void *a, const void *b,
                                 const void *c, void *d, size_t n, uint32_t smask,
-                                uint32_t *flags, int grid, hipStream_t st)
+                                uint32_t *flags, uint32_t slot_mask, int grid, hipStream_t st)
 {
     const bool arith = op >= OP_FMA && op <= OP_MUL;
+    const uint32_t sm = slot_mask;
     switch (fmt) {
-    case 0: return arith ? launch_arith<1>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
-                         : launch_misc<1>(op, a, b, c, d, n, smask, flags, grid, st);
-    case 1: return arith ? launch_arith<2>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
-                         : launch_misc<2>(op, a, b, c, d, n, smask, flags, grid, st);
-    case 2: return arith ? launch_arith<4>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
-                         : launch_misc<4>(op, a, b, c, d, n, smask, flags, grid, st);
-    default: return arith ? launch_arith<8>(op, a, b, c, d, n, smask, rnd, flags, grid, st)
-                          : launch_misc<8>(op, a, b, c, d, n, smask, flags, grid, st);
+    case 0: return arith ? launch_arith<1>(op, a, b, c, d, n, smask, rnd, flags, sm, grid, st)
+                         : launch_misc<1>(op, a, b, c, d, n, smask, flags, sm, grid, st);
+    case 1: return arith ? launch_arith<2>(op, a, b, c, d, n, smask, rnd, flags, sm, grid, st)
+                         : launch_misc<2>(op, a, b, c, d, n, smask, flags, sm, grid, st);
+    case 2: return arith ? launch_arith<4>(op, a, b, c, d, n, smask, rnd, flags, sm, grid, st)
+                         : launch_misc<4>(op, a, b, c, d, n, smask, flags, sm, grid, st);
+    default: return arith ? launch_arith<8>(op, a, b, c, d, n, smask, rnd, flags, sm, grid, st)
+                          : launch_misc<8>(op, a, b, c, d, n, smask, flags, sm, grid, st);
     }
 }
 
@@ -201,5 +224,12 @@ extern "C" hipError_t cftk_fill32(uint32_t *p, uint32_t v, size_t n, hipStream_t
 {
     const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
     hipLaunchKernelGGL(k_fill32, dim3(grid > 0 ? grid : 1), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cftk_flags_gather(uint32_t *slots, int nslots, uint32_t *host_word,
+                                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_flags_gather, dim3(1), dim3(64), 0, st, slots, nslots, host_word);
     return hipGetLastError();
 }

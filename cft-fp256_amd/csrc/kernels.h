@@ -10,16 +10,30 @@
 
 extern "C" {
 
+// Flag slots.  The elementwise kernels OR a wavefront's flags into slot
+// (workgroup index & slot_mask), the word at flags + slot * CFTK_SLOT_WORDS:
+// one 128-byte line a slot, so the waves of a short launch, which all finish
+// at once, spread their atomics over up to CFTK_FLAG_SLOTS lines instead of
+// queueing on one address.  slot_mask 0: the single word *flags.
+#define CFTK_FLAG_SLOTS 64
+#define CFTK_SLOT_WORDS 32
+
 // d[i] = op(a[i], b[i], c[i]) over n elements of format fmt (0..3).  An
 // operand the opcode does not read may be NULL; bit r of smask makes operand
-// r a single element.  The call's flags are ORed into *flags.
+// r a single element.  The call's flags are ORed into the slots at flags.
 hipError_t cftk_elem(int fmt, int op, int rnd, const void *a, const void *b, const void *c,
-                     void *d, size_t n, uint32_t smask, uint32_t *flags, int grid,
-                     hipStream_t st);
+                     void *d, size_t n, uint32_t smask, uint32_t *flags, uint32_t slot_mask,
+                     int grid, hipStream_t st);
 
 // Correctly-rounded a/b (op 0) or sqrt(a) (op 1).
 hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *b, void *d,
-                        size_t n, uint32_t smask, uint32_t *flags, int grid, hipStream_t st);
+                        size_t n, uint32_t smask, uint32_t *flags, uint32_t slot_mask, int grid,
+                        hipStream_t st);
+
+// The OR of the words of slots 0..nslots-1 (nslots <= CFTK_FLAG_SLOTS),
+// stored to *host_word (the device address of pinned host memory) with a
+// system-scope store; every slot is zero again afterwards.  One wavefront.
+hipError_t cftk_flags_gather(uint32_t *slots, int nslots, uint32_t *host_word, hipStream_t st);
 
 // One launch of the reductions' tree: for each of nseg segments of m
 // partials (segment s at src[s*m ..)), groups of sixteen consecutive partials
