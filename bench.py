@@ -568,21 +568,25 @@ def main():
     ops = {}
     if not args.skip_ops:
         for name in ("add", "mul", "div", "sqrt"):
+            def op_call(rnd, name=name):
+                if name == "add":
+                    return lib.cft_run(dev.handle, 1, 3, rnd, bufs[0].data_ptr, None,
+                                       bufs[2].data_ptr, bufs[3].data_ptr, n, None, None)
+                if name == "mul":
+                    return lib.cft_run(dev.handle, 3, 3, rnd, bufs[0].data_ptr,
+                                       bufs[1].data_ptr, None, bufs[3].data_ptr, n, None, None)
+                if name == "div":
+                    return lib.cft_div(dev.handle, 3, rnd, bufs[0].data_ptr, bufs[1].data_ptr,
+                                       bufs[3].data_ptr, n, None, None)
+                return lib.cft_sqrt(dev.handle, 3, rnd, bufs[0].data_ptr, bufs[3].data_ptr,
+                                    n, None, None)
+            # one untimed call first: a kernel's first launch also loads its
+            # code object, which would land between the timer's events
+            assert op_call(0) == 0, lib.cft_last_error()
             lib.cftx_kernel_timer(dev.handle, 1)
             t1 = time.perf_counter()
             for rnd in range(5):
-                if name == "add":
-                    st = lib.cft_run(dev.handle, 1, 3, rnd, bufs[0].data_ptr, None,
-                                     bufs[2].data_ptr, bufs[3].data_ptr, n, None, None)
-                elif name == "mul":
-                    st = lib.cft_run(dev.handle, 3, 3, rnd, bufs[0].data_ptr,
-                                     bufs[1].data_ptr, None, bufs[3].data_ptr, n, None, None)
-                elif name == "div":
-                    st = lib.cft_div(dev.handle, 3, rnd, bufs[0].data_ptr, bufs[1].data_ptr,
-                                     bufs[3].data_ptr, n, None, None)
-                else:
-                    st = lib.cft_sqrt(dev.handle, 3, rnd, bufs[0].data_ptr, bufs[3].data_ptr,
-                                      n, None, None)
+                st = op_call(rnd)
                 assert st == 0, lib.cft_last_error()
             dt = time.perf_counter() - t1
             lib.cftx_kernel_timer_read(dev.handle, ctypes.byref(kms), ctypes.byref(klaunch))
@@ -641,12 +645,18 @@ def main():
         f1 = ctypes.c_uint32()
         lib.cft_run(dev.handle, 0, 2, 0, b1[0].data_ptr, b1[1].data_ptr, b1[2].data_ptr,
                     b1[3].data_ptr, n1, ctypes.byref(f1), None)
-        lib.cftx_kernel_timer(dev.handle, 1)
+        # a call here is ~30 us, so the call-to-call rate is timed without
+        # the kernel timer (its two events and their read-back are ~8 us of
+        # it) and the kernel's duration in a second pass with it
         t1 = time.perf_counter()
         for _ in range(20):
             lib.cft_run(dev.handle, 0, 2, 0, b1[0].data_ptr, b1[1].data_ptr, b1[2].data_ptr,
                         b1[3].data_ptr, n1, ctypes.byref(f1), None)
         dt = time.perf_counter() - t1
+        lib.cftx_kernel_timer(dev.handle, 1)
+        for _ in range(20):
+            lib.cft_run(dev.handle, 0, 2, 0, b1[0].data_ptr, b1[1].data_ptr, b1[2].data_ptr,
+                        b1[3].data_ptr, n1, ctypes.byref(f1), None)
         lib.cftx_kernel_timer_read(dev.handle, ctypes.byref(kms), ctypes.byref(klaunch))
         lib.cftx_kernel_timer(dev.handle, 0)
         ks = kms.value / 1e3 / max(1, klaunch.value)

@@ -24,22 +24,34 @@ from . import Device, Program, _lib, odes, run_multi, workload
 
 
 def fma(args):
+    """binary256 (--fmt 3): the five rounding modes, one call each, over the
+    elementwise configuration's operands.  binary128 (--fmt 2): twenty RNE
+    calls over the binary128 configuration's operands (one untimed first)."""
     lib = _lib.lib()
     n = args.n
+    fmt = 2 if args.fmt == 2 else 3
+    esz = 16 if fmt == 2 else 32
     with Device("hip:0") as dev:
-        bufs = [dev.alloc(n * 32) for _ in range(4)]
-        workload.elementwise(2, n, seed=args.seed, threads=16,
+        bufs = [dev.alloc(n * esz) for _ in range(4)]
+        workload.elementwise(fmt - 1, n, seed=args.seed, threads=16,
                              out=(bufs[0].host, bufs[1].host, bufs[2].host))
         for b in bufs[:3]:
             b.to_device()
         fl = ctypes.c_uint32()
-        t0 = time.perf_counter()
-        for rnd in range(5):
-            _lib.check(lib.cft_run(dev.handle, 0, 3, rnd, bufs[0].data_ptr, bufs[1].data_ptr,
+        modes = [0] * 20 if fmt == 2 else list(range(5))
+
+        def call(rnd):
+            _lib.check(lib.cft_run(dev.handle, 0, fmt, rnd, bufs[0].data_ptr, bufs[1].data_ptr,
                                    bufs[2].data_ptr, bufs[3].data_ptr, n, ctypes.byref(fl),
                                    None), "cft_run")
+        if fmt == 2:
+            call(0)
+        t0 = time.perf_counter()
+        for rnd in modes:
+            call(rnd)
         dt = time.perf_counter() - t0
-        print(f"fma: {5 * n} elements in {dt:.4f} s ({5 * n / dt / 1e9:.3f} Gelem/s)")
+        k = len(modes) * n
+        print(f"fma: {k} elements in {dt:.4f} s ({k / dt / 1e9:.3f} Gelem/s), flags {fl.value}")
         for b in bufs:
             b.free()
 

@@ -156,14 +156,20 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
     if ((st = arena_reserve(dev, dev->stage[ROLE_RED0], first * esz, who)) != CFT_OK ||
         (st = arena_reserve(dev, dev->stage[ROLE_RED1], second * esz, who)) != CFT_OK)
         return st;
-    const size_t nwords = override94 ? 2 * nseg : 1;
-    if ((st = arena_reserve(dev, dev->stage[ROLE_RWORDS], nwords * 4, who)) != CFT_OK)
-        return st;
-    uint32_t *segflags = (uint32_t *)dev->stage[ROLE_RWORDS].p;
-    uint32_t *segspec = override94 ? segflags + nseg : nullptr;
+    // SUMSQ/SUMABS: a flag word and a specials word a segment, cleared here
+    // and read back.  The others: the call's flags through the handle's flag
+    // slots (zero between calls), gathered into the pinned host word.
+    const size_t nwords = override94 ? 2 * nseg : 0;
+    uint32_t *segflags = dev->dslots, *segspec = nullptr;
+    if (override94) {
+        if ((st = arena_reserve(dev, dev->stage[ROLE_RWORDS], nwords * 4, who)) != CFT_OK)
+            return st;
+        segflags = (uint32_t *)dev->stage[ROLE_RWORDS].p;
+        segspec = segflags + nseg;
+    }
     void *part[2] = {dev->stage[ROLE_RED0].p, dev->stage[ROLE_RED1].p};
 
-    hipError_t e = cftk_fill32(segflags, 0u, nwords, dev->stream);
+    hipError_t e = override94 ? cftk_fill32(segflags, 0u, nwords, dev->stream) : hipSuccess;
     if (e == hipSuccess)
         e = timer_begin(dev);
     const int leaf = op == OP_DOT ? 1 : op == OP_SUMABS ? 2 : op == OP_SUMSQ ? 3 : 0;
@@ -184,6 +190,8 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
     }
     if (e == hipSuccess)
         e = timer_end(dev);
+    if (e == hipSuccess && !override94)
+        e = cftk_flags_gather(dev->dslots, CFTK_FLAG_SLOTS, dev->hwords_dev, dev->stream);
     if (e != hipSuccess)
         return dev_fail(dev, e, who, "launching the reduction");
 
@@ -204,7 +212,7 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
         e = hipMemcpyAsync(res.data(), src, nseg * esz, hipMemcpyDeviceToHost, dev->stream);
     }
     std::vector<uint32_t> words(nwords);
-    if (e == hipSuccess)
+    if (e == hipSuccess && override94)
         e = hipMemcpyAsync(words.data(), segflags, nwords * 4, hipMemcpyDeviceToHost,
                            dev->stream);
     if (e == hipSuccess)
@@ -217,7 +225,7 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
 
     uint32_t acc = 0;
     if (!override94) {
-        acc = words[0];
+        acc = ((volatile uint32_t *)dev->hwords)[0];
     } else {
         // 754-2019 9.4: a NaN result over a segment holding an infinity is
         // +inf, with INVALID exactly when the segment holds a signalling NaN

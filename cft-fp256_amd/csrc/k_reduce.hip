@@ -132,11 +132,13 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
     const uint64_t dbase = sq * m + 16 * sr;
     const uint64_t wrap = m - 16 * m2;           // s + 1, g - m2 (mod 2^64)
     const bool by_seg = per_seg || segspec;
+    // one word a segment (per_seg), else the workgroup's flag slot
+    const uint64_t fslot = (uint64_t)(blockIdx.x & (CFTK_FLAG_SLOTS - 1)) * CFTK_SLOT_WORDS;
     uint64_t cur = s;
     uint32_t fl = 0, spec = 0;
     for (; o < total; o += stride) {
         if (by_seg && s != cur) {                // a new segment: the last one's bits out
-            seg_or(segflags, per_seg ? cur : 0, fl);
+            seg_or(segflags, per_seg ? cur : fslot, fl);
             if (segspec)
                 seg_or(segspec, cur, spec);
             fl = spec = 0;
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(256) void k_reduce(const uint32_t *__restrict__ src
             base += wrap;
         }
     }
-    seg_or(segflags, per_seg ? cur : 0, fl);
+    seg_or(segflags, per_seg ? cur : fslot, fl);
     if (segspec)
         seg_or(segspec, cur, spec);
 }

@@ -40,7 +40,8 @@ hipError_t cftk_flags_gather(uint32_t *slots, int nslots, uint32_t *host_word, h
 // become one node at dst[s*m2 + g], m2 = ceil(m/16).  leaf (the first launch
 // only): 0 the partial itself, 1 its rounded product with srcb's, 2 its
 // magnitude, 3 its rounded square; kind: 0 ADD nodes, 1 maximum nodes.  Flags are ORed into
-// segflags[s] (per_seg) or segflags[0]; segspec, when given, gets bit 0 where
+// segflags[s] (per_seg), else into the flag slots at segflags (CFTK_FLAG_SLOTS
+// of them, slot = workgroup & (CFTK_FLAG_SLOTS - 1)); segspec, when given, gets bit 0 where
 // a partial of the segment is an infinity and bit 1 where it is a signalling
 // NaN.
 hipError_t cftk_reduce(int fmt, const void *src, const void *srcb, void *dst, uint64_t nseg,
