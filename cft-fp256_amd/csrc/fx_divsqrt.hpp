@@ -229,39 +229,53 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         // 2^-63 (V B >= 2^64), V's only raise the estimate, and the sums and
         // the quotient (below 2^33) carry 6.5 * 2^-53 relative, 2^-17.3
         // absolute, of rounding; the digit itself is at most R / (V B).  So
-        // + 2^-15 never undershoots, and the steps down below run for the
-        // d^2 / (V B) < 1 the estimate ignores.
-        double dd = Rd / Vd + 0x1p-15;
+        // + 2^-15 never undershoots, and the add-back below runs for the
+        // d^2 / (V B) < 1 the estimate ignores: from the third digit on that
+        // is under 2^-32 (V B >= 2^96), but for the second (V B >= 2^64) it
+        // stepped about 30% of lanes, so nearly every wavefront.  There the
+        // digit is floor(x) for the root x of (V B + x) x = R, taken as
+        // x = 2R / (V B + sqrt(V^2 B^2 + 4R)), which has no cancellation: Rd
+        // is R with three roundings (all of R's limbs), Vd = V B exactly,
+        // and the square, the sum, the root, the sum and the quotient bring
+        // the whole to at most 8 * 2^-53 relative, under 2^-18 absolute for
+        // x < 2^32.  + 2^-15 again never undershoots.
+        double dd;
+        if (k == 1)
+            dd = 2.0 * Rd / (Vd + sqrt(Vd * Vd + 4.0 * Rd)) + 0x1p-15;
+        else
+            dd = Rd / Vd + 0x1p-15;
         dd = dd > 4294967295.0 ? 4294967295.0 : (dd < 0.0 ? 0.0 : dd);
         uint32_t d = (uint32_t)dd;
-        // T = (V * B + d) * d, k + 3 limbs
-        uint32_t T[NL + 2];
+        // R -= T, T = (V*B + d) * d: the limbs of V*B + d are [d, V0 .. Vk];
+        // their products with d are formed independently, assembled by one
+        // add-with-carry chain and subtracted by one borrow chain
+        uint32_t pl[NL + 1], ph[NL + 1];
 #pragma unroll
-        for (int i = 0; i < NL + 2; i++)
-            T[i] = 0;
+        for (int i = 0; i <= k + 1; i++) {
+            const uint64_t p = (uint64_t)(i == 0 ? d : V[i - 1]) * d;
+            pl[i] = (uint32_t)p;
+            ph[i] = (uint32_t)(p >> 32);
+        }
+        uint32_t br = 0;
         {
-            // (V*B + d) has limbs [d, V0, V1, .., Vk]
-            uint32_t carry = 0;
+            uint32_t c = 0;
 #pragma unroll
             for (int i = 0; i <= k + 1; i++) {
-                const uint32_t w = i == 0 ? d : V[i - 1];
-                const uint64_t p = (uint64_t)w * d + carry;
-                T[i] = (uint32_t)p;
-                carry = (uint32_t)(p >> 32);
+                uint32_t co, bo;
+                const uint32_t ti = addc(pl[i], i ? ph[i - 1] : 0u, c, &co);
+                c = co;
+                R[i] = subc(R[i], ti, br, &bo);
+                br = bo;
             }
-            if (k + 2 < NL + 2)
-                T[k + 2] = carry;
-        }
-        // while T > R: T -= (V*B + 2d - 1); d--
-        auto t_gt_r = [&]() {
-            int c = 0;
 #pragma unroll
-            for (int i = 0; i < NL + 2; i++)
-                c = T[i] != R[i] ? (T[i] > R[i] ? 1 : -1) : c;
-            return c > 0;
-        };
-        while (t_gt_r()) {
-            // S = V*B + 2d - 1 as limbs (d >= 1 here, since T > R >= 0)
+            for (int i = k + 2; i < NL + 2; i++) {
+                uint32_t bo;
+                R[i] = subc(R[i], i == k + 2 ? ph[k + 1] + c : 0u, br, &bo);
+                br = bo;
+            }
+        }
+        // while R < 0: R += V*B + 2d - 1 (= T(d) - T(d - 1)); d--
+        while (br) {
             uint32_t Sv[NL + 2];
             {
                 const uint64_t low = 2ull * d - 1ull;
@@ -274,24 +288,15 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
                     c >>= 32;
                 }
             }
-            uint32_t br = 0;
+            uint32_t c = 0;
 #pragma unroll
             for (int i = 0; i < NL + 2; i++) {
-                const uint64_t t = (uint64_t)T[i] - Sv[i] - br;
-                T[i] = (uint32_t)t;
-                br = (uint32_t)(t >> 63);
+                uint32_t co;
+                R[i] = addc(R[i], Sv[i], c, &co);
+                c = co;
             }
+            br = c ^ 1u;                         // no carry out: still negative
             d--;
-        }
-        // R -= T
-        {
-            uint32_t br = 0;
-#pragma unroll
-            for (int i = 0; i < NL + 2; i++) {
-                const uint64_t t = (uint64_t)R[i] - T[i] - br;
-                R[i] = (uint32_t)t;
-                br = (uint32_t)(t >> 63);
-            }
         }
         r[NL - 1 - k] = d;
     }
