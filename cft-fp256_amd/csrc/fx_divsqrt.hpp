@@ -34,27 +34,21 @@ __device__ __forceinline__ void long_div(uint32_t (&U)[2 * NL + 1], const uint32
     const double rinv = 1.0 / dtop;
 #pragma unroll
     for (int j = NL - 1; j >= 0; j--) {
-        // estimate from U[j+NL], U[j+NL-1], U[j+NL-2] against D's top two
-        uint32_t qhat;
-        if (U[j + NL] >= D[NL - 1]) {
-            qhat = 0xFFFFFFFFu;
-        } else {
-            const double num = (double)U[j + NL] * 18446744073709551616.0 +
-                               (double)U[j + NL - 1] * 4294967296.0 +
-                               (double)(j + NL - 2 >= 0 ? U[(j + NL - 2) < 0 ? 0 : j + NL - 2] : 0u);
-            // At or above the digit: num, dtop, rinv and the product each
-            // carry one rounding, so num * rinv is within 5 * 2^-53
-            // relative, 2^-18.7 absolute (the quotient is below 2^32), of
-            // U3/D2 (the top three remainder limbs over the top two divisor
-            // limbs), and the true digit is at most floor((U3 + 1)/D2),
-            // 2^-63 above U3/D2.  So + 2^-17 never undershoots, and
-            // overshoots only when U3/D2 lies within 2^-17 below an integer
-            // (or by Knuth's truncated-divisor bound): the add-back below
-            // almost never runs.
-            double qd = num * rinv + 0x1p-17;
-            qd = qd > 4294967295.0 ? 4294967295.0 : (qd < 0.0 ? 0.0 : qd);
-            qhat = (uint32_t)qd;
-        }
+        // estimate from U[j+NL], U[j+NL-1], U[j+NL-2] against D's top two.
+        // At or above the digit: num, dtop, rinv and the product each carry
+        // one rounding, so num * rinv is within 5 * 2^-53 relative, 2^-18.7
+        // absolute (the quotient is below 2^32 * (1 + 2^-31)), of U3/D2 (the
+        // top three remainder limbs over the top two divisor limbs), and the
+        // true digit is at most floor((U3 + 1)/D2), 2^-63 above U3/D2, and at
+        // most B - 1.  So + 2^-17, capped at B - 1, never undershoots, and
+        // overshoots only when U3/D2 lies within 2^-17 below an integer (or
+        // by Knuth's truncated-divisor bound, also for the cap): the add-back
+        // below almost never runs.  The estimate is never negative, so one
+        // minimum is the whole clamp.
+        const double num = (double)U[j + NL] * 18446744073709551616.0 +
+                           (double)U[j + NL - 1] * 4294967296.0 +
+                           (double)(j + NL - 2 >= 0 ? U[(j + NL - 2) < 0 ? 0 : j + NL - 2] : 0u);
+        uint32_t qhat = (uint32_t)__builtin_fmin(num * rinv + 0x1p-17, 4294967295.0);
         // U[j .. j+NL] -= qhat * D: the NL partial products independently,
         // their limbs assembled by one carry chain and subtracted by one
         // borrow chain
@@ -244,8 +238,7 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
             dd = 2.0 * Rd / (Vd + sqrt(Vd * Vd + 4.0 * Rd)) + 0x1p-15;
         else
             dd = Rd / Vd + 0x1p-15;
-        dd = dd > 4294967295.0 ? 4294967295.0 : (dd < 0.0 ? 0.0 : dd);
-        uint32_t d = (uint32_t)dd;
+        uint32_t d = (uint32_t)__builtin_fmin(dd, 4294967295.0);   // dd > 0
         // R -= T, T = (V*B + d) * d: the limbs of V*B + d are [d, V0 .. Vk];
         // their products with d are formed independently, assembled by one
         // add-with-carry chain and subtracted by one borrow chain
