@@ -70,6 +70,20 @@ def test_every_opcode_every_format(device, oracle, fmt):
             assert gf == wf, (fmt, op, rnd)
 
 
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_division_and_root_next_to_one(device, oracle, fmt):
+    """Quotients within a few ulps of 1 (quotient digits B-1, where the
+    digit estimate is capped) and the square roots of both operands."""
+    from test_oracle import _next_to
+    a, b = _next_to(np.random.default_rng(950 + fmt), fmt, 3000)
+    for x, y in ((a, b), (b, a)):
+        for rnd in range(5):
+            for op, yy in ((256, y), (257, None)):
+                want, wf = oracle_elem(oracle, fmt, op, rnd, x, yy, None)
+                got, gf = gpu(device, fmt, op, rnd, x, yy, None)
+                assert np.array_equal(got, want) and gf == wf, (fmt, op, rnd)
+
+
 @pytest.mark.parametrize("cfg,fmt", [(1, 2), (2, 3)])
 def test_workload_configs_all_ops(device, oracle, cfg, fmt):
     """Configs 1 and 2 (near-boundary FMA cases included) at 2^16."""

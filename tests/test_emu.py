@@ -36,6 +36,20 @@ def test_emu_matches_oracle(emu, oracle, fmt):
             assert gf == wf, (fmt, op, rnd)
 
 
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_division_and_root_next_to_one(emu, oracle, fmt):
+    """Quotients within a few ulps of 1 (digits B-1, the estimate's cap)
+    and the square roots of both operands."""
+    from test_oracle import _next_to
+    a, b = _next_to(np.random.default_rng(900 + fmt), fmt, 1500)
+    for x, y in ((a, b), (b, a)):
+        for rnd in range(5):
+            for op, yy in ((256, y), (257, None)):
+                want, wf = oracle_elem(oracle, fmt, op, rnd, x, yy, None)
+                got, gf = emu_elem(emu, fmt, op, rnd, x, yy, None)
+                assert np.array_equal(got, want) and gf == wf, (fmt, op, rnd)
+
+
 @pytest.mark.parametrize("name,fmt", [("fp128", 2), ("fp256", 3)])
 def test_emu_matches_golden(emu, golden, name, fmt):
     g = golden[name]

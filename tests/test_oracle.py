@@ -54,6 +54,29 @@ def _rand_enc(rng, fmt, n):
     return np.frombuffer(b"".join(out), dtype=np.uint8).copy()
 
 
+def _next_to(rng, fmt, n):
+    """(a, b) with b within four ulps of a (either side, a fifth of the b
+    significands then set to all ones): a/b and b/a lie next to 1, so the
+    long division's quotient digits are B-1 or 0 and its remainder's top
+    limb meets the divisor's; square roots of either are next to a power of
+    two's root."""
+    ew, mw, w = {0: (8, 23, 32), 1: (11, 52, 64), 2: (15, 112, 128), 3: (19, 236, 256)}[fmt]
+    esz, bias = w // 8, (1 << (ew - 1)) - 1
+    span = min(256, 1 << (ew - 2))
+    aa, bb = [], []
+    for i in range(n):
+        m = int.from_bytes(rng.bytes(esz), "little") % (1 << mw)
+        x = ((bias + int(rng.integers(-span, span))) << mw) | m
+        k = int(rng.integers(1, 5)) * (1 if i % 2 else -1)
+        y = x + k
+        if i % 5 == 0:
+            y |= (1 << mw) - 1
+        aa.append(x.to_bytes(esz, "little"))
+        bb.append(y.to_bytes(esz, "little"))
+    return (np.frombuffer(b"".join(aa), dtype=np.uint8).copy(),
+            np.frombuffer(b"".join(bb), dtype=np.uint8).copy())
+
+
 @pytest.mark.parametrize("fmt", [0, 1, 2, 3])
 def test_oracle_matches_reference_library(oracle, ref, fmt):
     """Every elementwise opcode the engine runs, all modes, 600 cases each."""
