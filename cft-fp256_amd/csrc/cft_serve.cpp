@@ -241,7 +241,7 @@ hipError_t launch_elem(Worker &w)
     for (size_t k = 0; e == hipSuccess && k < B.jobs.size(); k++) {
         const cftx_job &J = B.jobs[k]->job;
         SJob *j = B.jobs[k];
-        const int grid = grid_for(ex, J.n);
+        const int grid = grid_capped(ex, J.n);
         if (J.kind == CFTX_JOB_ELEM)
             e = cftk_elem(J.fmt, J.op, J.rnd, j->da, j->db, j->dc, j->dd, J.n, 0u, w.dflags + k,
                           0u, grid, ex->stream);

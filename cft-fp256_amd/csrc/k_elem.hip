@@ -5,9 +5,10 @@
 // with 16-byte vector loads and keeps it in VGPRs as 32-bit limbs.
 //
 // Replaces the reference's k_run / k_run256 / k_run128 (host/src/
-// backend_hip.cpp:484-665).  Launch shape: a grid-stride loop over a fixed
-// grid sized from the CU count; each wavefront ORs its flags into one of the
-// call's flag slots (kernels.h) with at most one atomic.
+// backend_hip.cpp:484-665).  Launch shape: a grid-stride loop, launched by
+// the library with one workgroup for every 256 elements (the serving loop
+// caps its grids); each wavefront ORs its flags into one of the call's flag
+// slots (kernels.h) with at most one atomic.
 #include "fx_arith.hpp"
 #include "fx_misc.hpp"
 #include "kernels.h"
