@@ -84,6 +84,34 @@ def test_division_and_root_next_to_one(device, oracle, fmt):
                 assert np.array_equal(got, want) and gf == wf, (fmt, op, rnd)
 
 
+def test_flags_do_not_carry_between_calls(device, oracle):
+    """A call's kernel ORs its flags into the handle's flag slots, which the
+    call's gather leaves zero: calls of 2^16 + 3 elements (257 workgroups,
+    every slot) and of 5 (one workgroup) alternate between operands that
+    raise flags and exact ones (1 * 1 + 0, 1 / 1), each call's result and
+    flags the oracle's."""
+    rng = np.random.default_rng(77)
+    one = (262143 << 236).to_bytes(32, "little")          # binary256 1.0
+
+    def exact(n):
+        x = np.frombuffer(one * n, dtype=np.uint8).copy()
+        return x, x.copy(), np.zeros(n * 32, dtype=np.uint8)
+
+    def raising(n):
+        return tuple(_rand(rng, 3, n) for _ in range(3))
+
+    big, small = (1 << 16) + 3, 5
+    for kind, n in (("raising", big), ("exact", small), ("raising", small), ("exact", big),
+                    ("raising", big), ("exact", big), ("exact", small)):
+        a, b, c = (raising if kind == "raising" else exact)(n)
+        for op in (0, 256):
+            cc = c if op == 0 else None
+            want, wf = oracle_elem(oracle, 3, op, 0, a, b, cc)
+            got, gf = gpu(device, 3, op, 0, a, b, cc)
+            assert np.array_equal(got, want) and gf == wf, (kind, n, op, gf, wf)
+            assert kind == "raising" or gf == 0, (kind, n, op, gf)
+
+
 @pytest.mark.parametrize("cfg,fmt", [(1, 2), (2, 3)])
 def test_workload_configs_all_ops(device, oracle, cfg, fmt):
     """Configs 1 and 2 (near-boundary FMA cases included) at 2^16."""

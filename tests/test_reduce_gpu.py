@@ -54,6 +54,30 @@ def test_whole_array_matches_oracle(device, oc, fmt):
             assert got[1] == want[1], (op, n, rnd, got[1], want[1])
 
 
+def test_flags_do_not_carry_between_calls(device, oc):
+    """SUM, DOT and MAXALL take their flags through the handle's flag slots,
+    zero between calls: calls of 2^18 + 5 elements (every slot) and of 7
+    alternate between values whose sums round and ones (exact sums), each
+    call's result and flags the oracle's."""
+    rng = np.random.default_rng(78)
+    one = (262143 << 236).to_bytes(32, "little")          # binary256 1.0
+    big, small = (1 << 18) + 5, 7
+    for kind, n in (("rounding", big), ("exact", small), ("rounding", small), ("exact", big),
+                    ("rounding", big), ("exact", big)):
+        if kind == "exact":
+            a = np.frombuffer(one * n, dtype=np.uint8).copy()
+            b = a.copy()
+        else:
+            a = red_values(rng, 3, n, special=0.0)
+            b = red_values(rng, 3, n, special=0.0)
+        for op in (24, 25, 31):
+            want = run_oc(oc, 3, op, 0, a, b, n)
+            got = gpu(device, 3, op, 0, a, b, n)
+            assert got[0].tobytes() == want[0].tobytes(), (kind, n, op)
+            assert got[1] == want[1], (kind, n, op, got[1], want[1])
+            assert kind == "rounding" or got[1] == 0, (kind, n, op, got[1])
+
+
 @pytest.mark.parametrize("fmt", [0, 1, 2, 3])
 def test_every_rounding_attribute(device, oc, fmt):
     # sums that round at nearly every node, so the attribute shows everywhere
