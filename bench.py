@@ -52,7 +52,7 @@ FMA_BYTES = 128          # 3 x 32 B in, 32 B out per element
 # HBM bytes an FMA element actually moves: rocprofv3 FETCH_SIZE (x2.000) +
 # WRITE_SIZE (x1.000) at 2^22 elements, each scaled by the same counter on a
 # plain 256 MiB copy (profiles/fma_fp256_elementwise.md)
-FMA_TRAFFIC_PER_ELEM = (393746.0 + 131353.0) * 1024 / 4194304
+FMA_TRAFFIC_PER_ELEM = (393543.0 + 131129.0) * 1024 / 4194304
 BYTES = {"add": 96, "mul": 96, "fma": 128, "div": 96, "sqrt": 64}
 
 
