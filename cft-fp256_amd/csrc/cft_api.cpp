@@ -224,14 +224,6 @@ int grid_for(const cft_device *dev, size_t n)
     return (int)std::max<size_t>(1, std::min<size_t>(want, (size_t)INT_MAX));
 }
 
-// The capped grid, 8 workgroups a CU at most, for launches whose flags go
-// to a single word (the serving loop's jobs): fewer waves to queue on it.
-int grid_capped(const cft_device *dev, size_t n)
-{
-    const size_t want = (n + 255) / 256;
-    const size_t cap = (size_t)dev->num_cus * 8;
-    return (int)std::max<size_t>(1, std::min(want, cap));
-}
 
 cft_buffer *find_buffer(cft_device *dev, const void *p, size_t bytes)
 {

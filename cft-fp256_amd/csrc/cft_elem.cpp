@@ -74,7 +74,7 @@ cft_status execute(const char *who, cft_device *dev, int which, int op, int fmt,
     if (e == hipSuccess)
         e = timer_end(dev);
     if (e == hipSuccess)
-        e = cftk_flags_gather(dev->dslots, nslots, dev->hwords_dev, dev->stream);
+        e = cftk_flags_gather(dev->dslots, nslots, dev->hwords_dev, 1, dev->stream);
     if (e != hipSuccess)
         return dev_fail(dev, e, who, "launching the kernel");
     if (!dbuf) {

@@ -142,4 +142,3 @@ cft_status device_open(int hip_index, uint32_t depth, cft_device **out, bool own
 
 // Kernel grid for n lanes of one element each.
 int grid_for(const cft_device *dev, size_t n);
-int grid_capped(const cft_device *dev, size_t n);

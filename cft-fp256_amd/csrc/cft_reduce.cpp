@@ -191,7 +191,7 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
     if (e == hipSuccess)
         e = timer_end(dev);
     if (e == hipSuccess && !override94)
-        e = cftk_flags_gather(dev->dslots, CFTK_FLAG_SLOTS, dev->hwords_dev, dev->stream);
+        e = cftk_flags_gather(dev->dslots, CFTK_FLAG_SLOTS, dev->hwords_dev, 1, dev->stream);
     if (e != hipSuccess)
         return dev_fail(dev, e, who, "launching the reduction");
 

@@ -116,8 +116,12 @@ struct Worker {
     std::vector<SJob *> set;             // admitted program jobs
     std::unordered_map<const cft_program *, size_t> held;   // jobs in the set, a program
     bool draining = false;               // admits nothing until its set is empty
-    uint32_t *dflags = nullptr;          // per-job flag words of an elementwise batch
+    // an elementwise batch's flags: a slot area a job (kernels.h), zero
+    // between batches, gathered into the pinned per-job words
+    uint32_t *dflags = nullptr;
     uint32_t *hflags = nullptr;          // pinned
+    uint32_t *hflags_dev = nullptr;      // hflags' device address
+    bool flags_dirty = false;            // a failed batch may have left bits
     uint8_t *hlf = nullptr;              // pinned lane flags of a whole batch
     size_t hlf_cap = 0;
     std::vector<uint8_t *> retired_host; // outgrown pinned buffers, freed at close
@@ -237,21 +241,28 @@ hipError_t launch_elem(Worker &w)
 {
     Batch &B = w.batch;
     cft_device *ex = w.dev;
-    hipError_t e = cftk_fill32(w.dflags, 0u, B.jobs.size(), ex->stream);
+    constexpr size_t area = (size_t)CFTK_FLAG_SLOTS * CFTK_SLOT_WORDS;
+    hipError_t e = hipSuccess;
+    if (w.flags_dirty) {
+        e = cftk_fill32(w.dflags, 0u, kMaxElemBatch * area, ex->stream);
+        if (e == hipSuccess)
+            w.flags_dirty = false;
+    }
     for (size_t k = 0; e == hipSuccess && k < B.jobs.size(); k++) {
         const cftx_job &J = B.jobs[k]->job;
         SJob *j = B.jobs[k];
-        const int grid = grid_capped(ex, J.n);
+        const int grid = grid_for(ex, J.n);
+        const uint32_t sm = grid > 1 ? CFTK_FLAG_SLOTS - 1 : 0u;
         if (J.kind == CFTX_JOB_ELEM)
-            e = cftk_elem(J.fmt, J.op, J.rnd, j->da, j->db, j->dc, j->dd, J.n, 0u, w.dflags + k,
-                          0u, grid, ex->stream);
+            e = cftk_elem(J.fmt, J.op, J.rnd, j->da, j->db, j->dc, j->dd, J.n, 0u,
+                          w.dflags + k * area, sm, grid, ex->stream);
         else
             e = cftk_divsqrt(J.fmt, J.kind == CFTX_JOB_DIV ? 0 : 1, J.rnd, j->da, j->db, j->dd,
-                             J.n, 0u, w.dflags + k, 0u, grid, ex->stream);
+                             J.n, 0u, w.dflags + k * area, sm, grid, ex->stream);
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(w.hflags, w.dflags, B.jobs.size() * 4, hipMemcpyDeviceToHost,
-                           ex->stream);
+    if (e == hipSuccess && !B.jobs.empty())
+        e = cftk_flags_gather(w.dflags, CFTK_FLAG_SLOTS, w.hflags_dev, (int)B.jobs.size(),
+                              ex->stream);
     return e;
 }
 
@@ -661,6 +672,7 @@ void start_elem(cftx_server *S, Worker &w)
         (void)hipGetLastError();
         fail_jobs(S, w.batch.jobs, CFT_ERR_INTERNAL);
         w.batch = Batch();
+        w.flags_dirty = true;                    // kernels may have run without the gather
         return;
     }
     w.busy = ELEM_BATCH;
@@ -899,9 +911,14 @@ extern "C" CFT_API cft_status cftx_server_open(cft_device *dev, int workers, siz
             if (st == CFT_OK && e == hipSuccess) e = hipEventCreate(&sl.t0);
             if (st == CFT_OK && e == hipSuccess) e = hipEventCreate(&sl.t1);
         }
-        if (st == CFT_OK && e == hipSuccess) e = hipMalloc(&w.dflags, kMaxElemBatch * 4);
+        const size_t fbytes = kMaxElemBatch * CFTK_FLAG_SLOTS * CFTK_SLOT_WORDS * 4;
+        if (st == CFT_OK && e == hipSuccess) e = hipMalloc(&w.dflags, fbytes);
+        if (st == CFT_OK && e == hipSuccess) e = hipMemsetAsync(w.dflags, 0, fbytes, w.dev->stream);
+        if (st == CFT_OK && e == hipSuccess) e = hipStreamSynchronize(w.dev->stream);
         if (st == CFT_OK && e == hipSuccess)
             e = hipHostMalloc((void **)&w.hflags, kMaxElemBatch * 4);
+        if (st == CFT_OK && e == hipSuccess)
+            e = hipHostGetDevicePointer((void **)&w.hflags_dev, w.hflags, 0);
         if (st == CFT_OK && e != hipSuccess) {
             (void)hipGetLastError();
             st = CFT_ERR_OUT_OF_MEMORY;

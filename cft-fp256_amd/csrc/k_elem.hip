@@ -6,9 +6,9 @@
 //
 // Replaces the reference's k_run / k_run256 / k_run128 (host/src/
 // backend_hip.cpp:484-665).  Launch shape: a grid-stride loop, launched by
-// the library with one workgroup for every 256 elements (the serving loop
-// caps its grids); each wavefront ORs its flags into one of the call's flag
-// slots (kernels.h) with at most one atomic.
+// the library with one workgroup for every 256 elements; each wavefront ORs
+// its flags into one of the call's flag slots (kernels.h) with at most one
+// atomic.
 #include "fx_arith.hpp"
 #include "fx_misc.hpp"
 #include "kernels.h"
@@ -143,12 +143,15 @@ __global__ void k_fill32(uint32_t *p, uint32_t v, size_t n)
         p[i] = v;
 }
 
-// One wavefront: lane s takes slot s's word and zeroes it, the wave ORs the
-// words, lane 0 stores the OR to host memory.  The store is system scope,
-// so it is in host memory when the stream's completion is seen.
+// One wavefront an area (workgroup b, area b): lane s takes slot s's word
+// and zeroes it, the wave ORs the words, lane 0 stores the OR to host
+// memory.  The store is system scope, so it is in host memory when the
+// stream's completion is seen.
 __global__ __launch_bounds__(64) void k_flags_gather(uint32_t *slots, int nslots,
-                                                     uint32_t *host_word)
+                                                     uint32_t *host_words)
 {
+    slots += (size_t)blockIdx.x * CFTK_FLAG_SLOTS * CFTK_SLOT_WORDS;
+    uint32_t *host_word = host_words + blockIdx.x;
     const int s = (int)threadIdx.x;
     uint32_t v = 0;
     if (s < nslots) {
@@ -228,9 +231,10 @@ extern "C" hipError_t cftk_fill32(uint32_t *p, uint32_t v, size_t n, hipStream_t
     return hipGetLastError();
 }
 
-extern "C" hipError_t cftk_flags_gather(uint32_t *slots, int nslots, uint32_t *host_word,
-                                        hipStream_t st)
+extern "C" hipError_t cftk_flags_gather(uint32_t *slots, int nslots, uint32_t *host_words,
+                                        int nareas, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_flags_gather, dim3(1), dim3(64), 0, st, slots, nslots, host_word);
+    hipLaunchKernelGGL(k_flags_gather, dim3(nareas), dim3(64), 0, st, slots, nslots,
+                       host_words);
     return hipGetLastError();
 }

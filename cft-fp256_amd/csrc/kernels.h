@@ -30,10 +30,13 @@ hipError_t cftk_divsqrt(int fmt, int which, int rnd, const void *a, const void *
                         size_t n, uint32_t smask, uint32_t *flags, uint32_t slot_mask, int grid,
                         hipStream_t st);
 
-// The OR of the words of slots 0..nslots-1 (nslots <= CFTK_FLAG_SLOTS),
-// stored to *host_word (the device address of pinned host memory) with a
-// system-scope store; every slot is zero again afterwards.  One wavefront.
-hipError_t cftk_flags_gather(uint32_t *slots, int nslots, uint32_t *host_word, hipStream_t st);
+// For each of nareas slot areas (area b at slots + b * CFTK_FLAG_SLOTS *
+// CFTK_SLOT_WORDS), the OR of the words of its slots 0..nslots-1 (nslots <=
+// CFTK_FLAG_SLOTS), stored to host_words[b] (the device address of pinned
+// host memory) with a system-scope store; every slot is zero again
+// afterwards.  One wavefront an area.
+hipError_t cftk_flags_gather(uint32_t *slots, int nslots, uint32_t *host_words, int nareas,
+                             hipStream_t st);
 
 // One launch of the reductions' tree: for each of nseg segments of m
 // partials (segment s at src[s*m ..)), groups of sixteen consecutive partials
