@@ -336,6 +336,19 @@ __device__ __forceinline__ void round_pack_norm(int sign, uint32_t (&K)[NL], boo
         flags |= inx ? FL_NX : 0u;
         return;
     }
+    // P + 1 or more places below 2^emin: K.g shifts out whole, into the
+    // sticky, so the result is +-0 or the least subnormal, tiny and inexact
+    // (the general path below gives the same, through a shift that a wave
+    // then skips unless one of its lanes lies within the subnormal range).
+    if (Fm::EMIN - e_norm >= Fm::P + 1) {
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+            out[i] = 0;
+        out[0] = round_incr(rnd, sign, false, true, false) ? 1u : 0u;
+        out[NL - 1] |= (uint32_t)sign << 31;
+        flags |= FL_NX | FL_UF;
+        return;
+    }
     // Below 2^emin before rounding: tiny unless rounding at full precision
     // carries exactly up to 2^emin.
     bool allones = (K[NL - 1] | ~((2u << Fm::TOPF) - 1u)) == 0xFFFFFFFFu;
