@@ -189,6 +189,7 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         R[1] = (uint32_t)(rem >> 32);
         r[NL - 1] = (uint32_t)d;                 // r kept left-aligned: digits fill down
     }
+    double rinv = 0.0;                           // 1 / (V B), set at the third digit
 #pragma unroll
     for (int k = 1; k < NL; k++) {
         // r so far: digits r[NL-k .. NL-1] (k limbs); R < 2^33 * 2^(32(k-1))
@@ -215,10 +216,12 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         for (int i = 0; i < NL + 2; i++)
             if (i <= k + 2 && i >= k - 1)
                 Rd += ldexp((double)R[i], 32 * i);
+        if (k <= 2) {
 #pragma unroll
-        for (int i = 0; i <= NL; i++)
-            if (i <= k && i >= k - 2)
-                Vd += ldexp((double)V[i], 32 * (i + 1));
+            for (int i = 0; i <= NL; i++)
+                if (i <= k)
+                    Vd += ldexp((double)V[i], 32 * (i + 1));
+        }
         // At or above the digit: R's dropped limbs move R / (V B) by under
         // 2^-63 (V B >= 2^64), V's only raise the estimate, and the sums and
         // the quotient (below 2^33) carry 6.5 * 2^-53 relative, 2^-17.3
@@ -233,11 +236,20 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         // and the square, the sum, the root, the sum and the quotient bring
         // the whole to at most 8 * 2^-53 relative, under 2^-18 absolute for
         // x < 2^32.  + 2^-15 again never undershoots.
+        // From the fourth digit on, V B is the third digit's Vd times
+        // 2^(32(k-2)), its reciprocal the third's times 2^(-32(k-2)): that drops V's
+        // limbs below (each digit's 2d, under 2^-63 of V B), which only
+        // raises the estimate, by under 2^-30; Rd * rinv carries at most
+        // 8 * 2^-53 relative of rounding (Rd three, Vd two, the reciprocal,
+        // the product, the bias sum), 2^-17 absolute, and + 2^-15 never
+        // undershoots.
         double dd;
-        if (k == 1)
+        if (k == 1) {
             dd = 2.0 * Rd / (Vd + sqrt(Vd * Vd + 4.0 * Rd)) + 0x1p-15;
-        else
-            dd = Rd / Vd + 0x1p-15;
+        } else {
+            rinv = k == 2 ? 1.0 / Vd : rinv * 0x1p-32;
+            dd = Rd * rinv + 0x1p-15;
+        }
         uint32_t d = (uint32_t)__builtin_fmin(dd, 4294967295.0);   // dd > 0
         // R -= T, T = (V*B + d) * d: the limbs of V*B + d are [d, V0 .. Vk];
         // their products with d are formed independently, assembled by one
