@@ -161,10 +161,12 @@ class Pools:
         # ODE programs, banks and states
         self.progs = {}
         self.banks = {}
-        self.progs["l63"] = Program(dev, P.with_trips(P.lorenz63(3), 100))
+        self.images = {"l63": P.with_trips(P.lorenz63(3), 100),
+                       "hh": P.with_trips(P.henonheiles(3), 100)}
+        self.progs["l63"] = Program(dev, self.images["l63"])
         self.banks["l63"] = np.frombuffer(P.bank_bytes("lorenz63-rk4", 3, "0.001"),
                                           dtype=np.uint8).copy()
-        self.progs["hh"] = Program(dev, P.with_trips(P.henonheiles(3), 100))
+        self.progs["hh"] = Program(dev, self.images["hh"])
         self.banks["hh"] = np.frombuffer(P.bank_bytes("henonheiles-lf", 3, "0.01"),
                                          dtype=np.uint8).copy()
         self.banks["l96"] = np.frombuffer(P.bank_bytes("lorenz96-rk4", 3, "0.01"),
@@ -177,7 +179,8 @@ class Pools:
                 (_k, _i, _b, _ids, st, steps), = odes.lorenz96(1, seed=r["seed"], fmt=3,
                                                                ks=[r["k"]])
                 if r["k"] not in self.progs:
-                    self.progs[r["k"]] = Program(dev, P.with_trips(P.lorenz96(3, k=r["k"]), 20))
+                    self.images[r["k"]] = P.with_trips(P.lorenz96(3, k=r["k"]), 20)
+                    self.progs[r["k"]] = Program(dev, self.images[r["k"]])
             elif r["kind"] == 5:
                 _i, _b, st, steps = odes.henonheiles(16, seed=r["seed"], fmt=3)
             else:
@@ -195,6 +198,9 @@ class Pools:
 
     def prog_of(self, r):
         return {3: self.progs["l63"], 5: self.progs["hh"]}.get(r["kind"]) or self.progs[r["k"]]
+
+    def image_of(self, r):
+        return {3: self.images["l63"], 5: self.images["hh"]}.get(r["kind"]) or self.images[r["k"]]
 
     def bank_of(self, r):
         return {3: self.banks["l63"], 4: self.banks["l96"], 5: self.banks["hh"]}[r["kind"]]

@@ -224,6 +224,46 @@ int grid_for(const cft_device *dev, size_t n)
     return (int)std::max<size_t>(1, std::min<size_t>(want, (size_t)INT_MAX));
 }
 
+cft_status device_window(const char *who, cft_device *dev, const char *what, const void *p,
+                        size_t bytes, size_t align)
+{
+    if ((uintptr_t)p % align) {
+        cft_set_error("%s: %s (%p) is not aligned to %zu bytes", who, what, p, align);
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    hipPointerAttribute_t at;
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess || at.type != hipMemoryTypeDevice || at.device != dev->hip_index) {
+        (void)hipGetLastError();
+        cft_set_error("%s: %s (%p) is not device memory of GPU %d", who, what, p,
+                      dev->hip_index);
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+    if (e != hipSuccess || (const uint8_t *)p + bytes > (const uint8_t *)base + size) {
+        (void)hipGetLastError();
+        cft_set_error("%s: %s's %zu bytes at %p run past the end of its allocation", who, what,
+                      bytes, p);
+        return CFT_ERR_INVALID_ARGUMENT;
+    }
+    return CFT_OK;
+}
+
+// host/src/softfloat.c:822-848: FMA and SELECT read all three, ADD/SUB a and
+// c, the sign operations and the seeds a alone, every other assigned opcode
+// a and b.
+unsigned cft_op_reads(int op)
+{
+    const bool assigned = (op >= 0 && op <= 14) || (op >= 16 && op <= 31);
+    switch (op) {
+    case 0: case 11: return 7u;
+    case 1: case 2: return 5u;
+    case 4: case 5: case 26: case 27: return 1u;
+    default: return assigned ? 3u : 0u;
+    }
+}
 
 cft_buffer *find_buffer(cft_device *dev, const void *p, size_t bytes)
 {

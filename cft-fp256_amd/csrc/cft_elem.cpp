@@ -21,24 +21,7 @@ namespace {
 
 bool is_reduction(int op) { return op == 24 || op == 25 || op == 28 || op == 29 || op == 31; }
 
-bool op_assigned(int op)
-{
-    return (op >= 0 && op <= 14) || (op >= 16 && op <= 23) || (op >= 24 && op <= 29) ||
-           op == 30 || op == 31;
-}
-
-// host/src/softfloat.c:822-848: which of a, b, c an opcode reads.
-unsigned op_operands(int op)
-{
-    switch (op) {
-    case 0: return 7u;
-    case 1: case 2: return 5u;
-    case 3: return 3u;
-    case 4: case 5: case 26: case 27: return 1u;
-    case 11: return 7u;
-    default: return op_assigned(op) ? 3u : 0u;
-    }
-}
+unsigned op_operands(int op) { return cft_op_reads(op); }
 
 // Everything after argument checking: bind, launch, read back.  which:
 // -1 an elementwise opcode, 0 div, 1 sqrt.
@@ -59,6 +42,28 @@ cft_status execute(const char *who, cft_device *dev, int which, int op, int fmt,
         (st = bind_in(dev, ROLE_C, c, nc * esz, &dc, who)) != CFT_OK ||
         (st = bind_out(dev, ROLE_D, d, n * esz, &dd, &dbuf, who)) != CFT_OK)
         return st;
+    // Inside one resident buffer an operand can share device memory with d.
+    // The same window (each element read before it is written) is safe; a
+    // window that overlaps d's otherwise - shifted by some elements, or a
+    // scalar inside it - would race across threads, so it is copied into
+    // its role's arena first and the call sees the bytes from before it, as
+    // it does for operands staged from host memory.
+    const uint8_t *d0 = (const uint8_t *)dd, *d1 = d0 + n * esz;
+    const void **din[3] = {&da, &db, &dc};
+    const size_t nin[3] = {na, nb, nc};
+    for (int r = 0; r < 3; r++) {
+        const uint8_t *p0 = (const uint8_t *)*din[r], *p1 = p0 + nin[r] * esz;
+        if (!p0 || p1 <= d0 || p0 >= d1 || (p0 == d0 && nin[r] == n))
+            continue;
+        DevArena &ar = dev->stage[ROLE_A + r];
+        if ((st = arena_reserve(dev, ar, nin[r] * esz, who)) != CFT_OK)
+            return st;
+        const hipError_t ce = hipMemcpyAsync(ar.p, p0, nin[r] * esz, hipMemcpyDeviceToDevice,
+                                             dev->stream);
+        if (ce != hipSuccess)
+            return dev_fail(dev, ce, who, "copying an operand that overlaps d");
+        *din[r] = ar.p;
+    }
     // The kernel ORs its flags into the handle's flag slots (zero between
     // calls); the gather launch after it ORs the slots into the pinned host
     // word and zeroes them again: no clearing launch before, no copy after.

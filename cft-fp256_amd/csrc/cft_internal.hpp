@@ -142,3 +142,20 @@ cft_status device_open(int hip_index, uint32_t depth, cft_device **out, bool own
 
 // Kernel grid for n lanes of one element each.
 int grid_for(const cft_device *dev, size_t n);
+
+// A window [p, p + bytes) of raw device memory (not a resident buffer):
+// aligned to `align` bytes, device memory of dev's GPU, and inside one
+// allocation.  CFT_ERR_INVALID_ARGUMENT with a sentence naming `what`
+// otherwise.
+cft_status device_window(const char *who, cft_device *dev, const char *what, const void *p,
+                         size_t bytes, size_t align);
+
+// cft_reduce's argument checks (host/src/device.c:2759-2862) in their order:
+// dev, format, rounding attribute, a reduction opcode, d, and for n > 0 the
+// operands the opcode reads and a byte count a size_t holds.
+cft_status reduce_check(const char *who, cft_device *dev, int op, int fmt, int rnd,
+                        const void *a, const void *b, void *d, size_t n);
+
+// Which of a, b, c (bits 0, 1, 2) elementwise opcode op reads
+// (host/src/softfloat.c:822-848); 0 for an unassigned opcode.
+unsigned cft_op_reads(int op);

@@ -256,6 +256,16 @@ cft_status execute(const char *who, cft_device *dev, int op, int fmt, int rnd, c
 
 } // namespace
 
+// cft_reduce's argument rules in its order, for its variants.
+cft_status reduce_check(const char *who, cft_device *dev, int op, int fmt, int rnd,
+                        const void *a, const void *b, void *d, size_t n)
+{
+    cft_status st = check_head(who, dev, op, fmt, rnd, d, false);
+    if (st == CFT_OK && n)
+        st = check_operands(who, op, a, b, n, cft_format_size((cft_format)fmt));
+    return st;
+}
+
 // reference host/include/cft.h:1284-1293, host/src/device.c:2731-2757
 extern "C" CFT_API cft_status cft_reduce(cft_device *dev, cft_op op, cft_format fmt,
                                          cft_round rnd, const void *a, const void *b, void *d,

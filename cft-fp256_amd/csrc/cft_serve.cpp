@@ -55,6 +55,7 @@ struct SJob {
     size_t footprint = 0;
     cftx_job_result res;
     bool done = false;
+    bool claimed = false;                // a cftx_server_wait holds this ticket
     // a program job in a running set
     bool pure = false;
     std::vector<uint32_t> left;          // trips to go, a lane
@@ -690,9 +691,11 @@ void start_whole(cftx_server *S, Worker &w, SJob *first)
         SJob *j = *it;
         if (S->max_jobs && B.jobs.size() >= S->max_jobs)
             break;
+        // the first job is always taken, whatever its size; later ones join
+        // while the batch stays within 2^22 lanes and the memory budget
         if (j->job.kind == CFTX_JOB_PROGRAM && j->job.prog == prog && j->bank == bank &&
-            lanes + j->job.n <= (1u << 22) &&
-            (B.jobs.empty() || S->in_use + j->footprint <= S->budget)) {
+            (B.jobs.empty() ||
+             (lanes + j->job.n <= (1u << 22) && S->in_use + j->footprint <= S->budget))) {
             B.jobs.push_back(j);
             lanes += j->job.n;
             it = S->waiting.erase(it);
@@ -703,6 +706,8 @@ void start_whole(cftx_server *S, Worker &w, SJob *first)
             ++it;
         }
     }
+    if (B.jobs.empty())
+        return;
     w.batch = std::move(B);
     cft_status st = job_bank(S, w.batch.jobs[0]);
     for (SJob *j : w.batch.jobs)
@@ -960,13 +965,18 @@ extern "C" CFT_API cft_status cftx_server_submit(cftx_server *S, const cftx_job 
     j->job = *job;
     memset(&j->res, 0, sizeof j->res);
     j->res.struct_size = sizeof j->res;
-    // operands: windows of current resident buffers
+    // operands: windows of current resident buffers, or of device memory of
+    // the server's GPU that the caller allocated (aligned for the kernels'
+    // loads, inside one allocation)
     auto window = [&](const void *p, size_t bytes, const char *what) -> void * {
         cft_buffer *b = find_buffer(dev, p, bytes);
         if (!b) {
-            cft_set_error("%s: %s is not a window of a resident buffer of the server's device",
-                          who, what);
-            return nullptr;
+            const size_t esz = job->kind == CFTX_JOB_PROGRAM && job->prog
+                                   ? job->prog->esz
+                                   : cft_format_size((cft_format)job->fmt);
+            if (device_window(who, dev, what, p, bytes, esz < 16 ? esz : 16) != CFT_OK)
+                return nullptr;
+            return const_cast<void *>(p);
         }
         if (!b->dev_valid) {
             cft_set_error("%s: %s's buffer is not current on the device; call "
@@ -1000,12 +1010,13 @@ extern "C" CFT_API cft_status cftx_server_submit(cftx_server *S, const cftx_job 
                 st = CFT_ERR_INVALID_ARGUMENT;
             else
                 mark_out(J.d, bytes);
-            // ADD/SUB read c, MUL b, FMA all three: a NULL one the opcode reads
-            // is refused as cft_run refuses it
-            const int op = J.op;
+            // a NULL operand the opcode reads is refused exactly as cft_run
+            // refuses it (the same table of which operands an opcode reads)
+            const unsigned reads = cft_op_reads(J.op);
             if (st == CFT_OK && J.kind == CFTX_JOB_ELEM &&
-                (((op == 0 || op == 3) && !J.b) || ((op == 0 || op == 1 || op == 2) && !J.c))) {
-                cft_set_error("%s: opcode %d reads an operand that is NULL", who, op);
+                (((reads & 2u) && !J.b) || ((reads & 4u) && !J.c))) {
+                cft_set_error("%s: opcode %d (%s) reads operand %c, which is NULL", who, J.op,
+                              cft_op_name((cft_op)J.op), ((reads & 2u) && !J.b) ? 'b' : 'c');
                 st = CFT_ERR_INVALID_ARGUMENT;
             }
         }
@@ -1069,19 +1080,25 @@ extern "C" CFT_API cft_status cftx_server_wait(cftx_server *S, uint64_t ticket,
     }
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->jobs.find(ticket);
-    if (it == S->jobs.end()) {
-        cft_set_error("cftx_server_wait: ticket %llu is not an outstanding job",
+    if (it == S->jobs.end() || it->second->claimed) {
+        cft_set_error(it == S->jobs.end()
+                          ? "cftx_server_wait: ticket %llu is not an outstanding job"
+                          : "cftx_server_wait: ticket %llu is already being waited for",
                       (unsigned long long)ticket);
         return CFT_ERR_INVALID_ARGUMENT;
     }
+    // the job stays in the table while this thread sleeps (submissions may
+    // rehash it, so no iterator is kept across the wait); a second waiter
+    // on the same ticket is refused above instead of touching a freed job
     SJob *j = it->second;
+    j->claimed = true;
     S->cv_done.wait(lk, [&] { return j->done; });
     const size_t want = std::min(out->struct_size ? out->struct_size : sizeof *out,
                                  sizeof(cftx_job_result));
     cftx_job_result r = j->res;
     r.struct_size = want;
     memcpy(out, &r, want);
-    S->jobs.erase(it);
+    S->jobs.erase(ticket);
     delete j;
     return CFT_OK;
 }

@@ -161,8 +161,11 @@ CFT_API cft_status cftx_program_prepare(cft_program *prog, uint32_t widths, uint
  *    steps.  Program jobs of other programs run whole, one batch a program.
  * Operands, outputs and states are windows of resident buffers (cft_alloc)
  * of the device the server was opened on, made current on the device
- * (cft_buffer_to_device) before submission; states are advanced in place and
- * results stay on the device.  Each job's result is the bits and flags the
+ * (cft_buffer_to_device) before submission, or windows of device memory of
+ * that GPU the caller allocated itself (aligned to min(element size, 16)
+ * bytes, inside one allocation); states are advanced in place and results
+ * stay on the device.  An impure program job (one the wide runner does not
+ * take) runs whole, alone when it has more than 2^22 lanes.  Each job's result is the bits and flags the
  * same job gives alone.  No reference counterpart (host/tools/cft-serve.c
  * answers one request at a time). */
 #define CFTX_JOB_ELEM    0   /* d = op(a, b, c): cft_run's opcodes */
@@ -176,12 +179,12 @@ typedef struct cftx_job {
     size_t          struct_size;
     int             kind;
     int             op, fmt, rnd;        /* elementwise */
-    const void     *a, *b, *c;           /* resident windows, n elements (NULL: unread) */
+    const void     *a, *b, *c;           /* device windows, n elements (NULL: unread) */
     void           *d;
     size_t          n;                   /* elements, or lanes of a program job */
     cft_program    *prog;                /* program jobs: loaded on the server's device */
     const void     *bank;  size_t bank_bytes;   /* host bytes, copied at submission */
-    void           *state; size_t state_bytes;  /* a resident window, n x slots elements */
+    void           *state; size_t state_bytes;  /* a device window, n x slots elements */
     const uint32_t *steps;               /* n entries, copied at submission */
 } cftx_job;
 
