@@ -240,6 +240,7 @@ def _bind(lib):
         "cftx_server_set_batching": ([V, c_uint32], c_int),
         "cftx_server_set_slice": ([V, ctypes.c_double], c_int),
         "cftx_buffer_device_ptr": ([V, V, c_size_t], c_void_p),
+        "cftx_reduce_noalloc": ([V, c_int, c_int, c_int, V, V, V, c_size_t, P, P], c_int),
         "cftx_program_wide": ([V, c_size_t, c_uint32, c_uint32, c_uint32, V, c_size_t,
                                POINTER(c_uint32)], c_int),
         "cft_lower_flags": ([V, c_uint32], None),

@@ -234,6 +234,23 @@ CFT_API void       cftx_server_close(cftx_server *srv);
  * other device code); NULL when p is not inside one of dev's buffers. */
 CFT_API void *cftx_buffer_device_ptr(cft_device *dev, const void *p, size_t bytes);
 
+/* cft_reduce (whole array) over operands already in device memory, with no
+ * allocation at all.  No counterpart in the reference: it is for callers
+ * that cannot allocate - code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  a, b (CFT_DOT only) and d are device
+ * addresses of the handle's GPU (cftx_buffer_device_ptr, or the caller's own
+ * device memory), aligned to min(element size, 16) bytes and each within one
+ * allocation; the result is written to d on the device.  Arguments, results
+ * and flags are cft_reduce's.  The tree's partial results live in a scratch
+ * array of 8.5 MiB of device memory a GPU that the library holds until the
+ * process ends, so n is at most 2^22 binary256, 2^23 binary128, 2^24 binary64
+ * or 2^25 binary32 elements (CFT_ERR_INVALID_ARGUMENT beyond).  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (two handles,
+ * two threads) race, and their results are undefined. */
+CFT_API cft_status cftx_reduce_noalloc(cft_device *dev, cft_op op, cft_format fmt,
+                                       cft_round rnd, const void *a, const void *b, void *d,
+                                       size_t n, uint32_t *flags_out, uint32_t *bus_out);
+
 #ifdef __cplusplus
 }
 #endif
