@@ -296,6 +296,132 @@ CFT_API void     cft_restore_flags(cft_device *dev, uint32_t saved,
                                    uint32_t mask);
 CFT_API int      cft_test_saved_flags(uint32_t saved, uint32_t mask);
 
+/* ---- entry points off this path ----------------------------------------
+ *
+ * The reference's Python binding (bindings/python/cftmpfr/_lib.py:181-376,
+ * `_bind`) declares every entry point below when it loads a library, so
+ * they are exported here with the reference's signatures; the operations
+ * themselves (transcendentals, the integer-exponent powers, augmented
+ * arithmetic, scaled products, formatOf, character conversions, NaN
+ * payloads, the 9.6 magnitude forms) are not on this library's path.  Each
+ * clears the error slot, sets one sentence naming itself and returns
+ * CFT_ERR_UNSUPPORTED, writing nothing.  A caller that needs them keeps the
+ * reference's software library for those calls. */
+
+/* reference host/include/cft.h:1648-1671 (one operand) */
+#define CFT_OFFPATH_UNARY(name) \
+    CFT_API cft_status name(cft_device *dev, cft_format fmt, cft_round rnd, \
+                            const void *a, void *d, size_t n, uint32_t *flags_out);
+CFT_OFFPATH_UNARY(cft_exp) CFT_OFFPATH_UNARY(cft_expm1) CFT_OFFPATH_UNARY(cft_exp2)
+CFT_OFFPATH_UNARY(cft_log) CFT_OFFPATH_UNARY(cft_log1p) CFT_OFFPATH_UNARY(cft_log2)
+CFT_OFFPATH_UNARY(cft_log10) CFT_OFFPATH_UNARY(cft_sinpi) CFT_OFFPATH_UNARY(cft_cospi)
+CFT_OFFPATH_UNARY(cft_tanpi) CFT_OFFPATH_UNARY(cft_asin) CFT_OFFPATH_UNARY(cft_acos)
+CFT_OFFPATH_UNARY(cft_atan) CFT_OFFPATH_UNARY(cft_asinpi) CFT_OFFPATH_UNARY(cft_acospi)
+CFT_OFFPATH_UNARY(cft_atanpi) CFT_OFFPATH_UNARY(cft_sin) CFT_OFFPATH_UNARY(cft_cos)
+CFT_OFFPATH_UNARY(cft_tan) CFT_OFFPATH_UNARY(cft_sinh) CFT_OFFPATH_UNARY(cft_cosh)
+CFT_OFFPATH_UNARY(cft_tanh) CFT_OFFPATH_UNARY(cft_asinh) CFT_OFFPATH_UNARY(cft_acosh)
+CFT_OFFPATH_UNARY(cft_atanh) CFT_OFFPATH_UNARY(cft_exp2m1) CFT_OFFPATH_UNARY(cft_exp10)
+CFT_OFFPATH_UNARY(cft_exp10m1) CFT_OFFPATH_UNARY(cft_log2p1) CFT_OFFPATH_UNARY(cft_log10p1)
+CFT_OFFPATH_UNARY(cft_rsqrt)
+#undef CFT_OFFPATH_UNARY
+
+/* reference host/include/cft.h:1672-1690 (two operands) */
+#define CFT_OFFPATH_BINARY(name) \
+    CFT_API cft_status name(cft_device *dev, cft_format fmt, cft_round rnd, \
+                            const void *a, const void *b, void *d, size_t n, \
+                            uint32_t *flags_out);
+CFT_OFFPATH_BINARY(cft_pow) CFT_OFFPATH_BINARY(cft_hypot) CFT_OFFPATH_BINARY(cft_atan2)
+CFT_OFFPATH_BINARY(cft_atan2pi) CFT_OFFPATH_BINARY(cft_powr)
+#undef CFT_OFFPATH_BINARY
+
+/* reference host/include/cft.h:2119-2130 (an integer exponent array) */
+CFT_API cft_status cft_pown(cft_device *dev, cft_format fmt, cft_round rnd,
+                            const void *a, const int64_t *n, void *d,
+                            size_t count, uint32_t *flags_out);
+CFT_API cft_status cft_compound(cft_device *dev, cft_format fmt, cft_round rnd,
+                                const void *a, const int64_t *n, void *d,
+                                size_t count, uint32_t *flags_out);
+CFT_API cft_status cft_rootn(cft_device *dev, cft_format fmt, cft_round rnd,
+                             const void *a, const int64_t *n, void *d,
+                             size_t count, uint32_t *flags_out);
+
+/* reference host/include/cft.h:2335-2348 (9.4 scaled products) */
+CFT_API cft_status cft_scaled_prod(cft_device *dev, cft_format fmt, cft_round rnd,
+                                   const void *a, void *pr, int64_t *scale_out,
+                                   size_t n, uint32_t *flags_out);
+CFT_API cft_status cft_scaled_prod_sum(cft_device *dev, cft_format fmt, cft_round rnd,
+                                       const void *a, const void *b, void *pr,
+                                       int64_t *scale_out, size_t n, uint32_t *flags_out);
+CFT_API cft_status cft_scaled_prod_diff(cft_device *dev, cft_format fmt, cft_round rnd,
+                                        const void *a, const void *b, void *pr,
+                                        int64_t *scale_out, size_t n, uint32_t *flags_out);
+
+/* reference host/include/cft.h:2440-2451 (9.5 augmented arithmetic) */
+CFT_API cft_status cft_augmented_add(cft_device *dev, cft_format fmt, const void *a,
+                                     const void *b, void *r, void *e, size_t n,
+                                     uint32_t *flags_out);
+CFT_API cft_status cft_augmented_sub(cft_device *dev, cft_format fmt, const void *a,
+                                     const void *b, void *r, void *e, size_t n,
+                                     uint32_t *flags_out);
+CFT_API cft_status cft_augmented_mul(cft_device *dev, cft_format fmt, const void *a,
+                                     const void *b, void *r, void *e, size_t n,
+                                     uint32_t *flags_out);
+
+/* reference host/include/cft.h:3511-3541 (5.4.1 formatOf) */
+#define CFT_OFFPATH_FORMATOF2(name) \
+    CFT_API cft_status name(cft_device *dev, cft_format sfmt, cft_format dfmt, \
+                            cft_round rnd, const void *a, const void *b, void *d, \
+                            size_t n, uint32_t *flags_out, uint32_t *bus_out);
+CFT_OFFPATH_FORMATOF2(cft_formatof_add) CFT_OFFPATH_FORMATOF2(cft_formatof_sub)
+CFT_OFFPATH_FORMATOF2(cft_formatof_mul) CFT_OFFPATH_FORMATOF2(cft_formatof_div)
+#undef CFT_OFFPATH_FORMATOF2
+CFT_API cft_status cft_formatof_sqrt(cft_device *dev, cft_format sfmt, cft_format dfmt,
+                                     cft_round rnd, const void *a, void *d, size_t n,
+                                     uint32_t *flags_out, uint32_t *bus_out);
+CFT_API cft_status cft_formatof_fma(cft_device *dev, cft_format sfmt, cft_format dfmt,
+                                    cft_round rnd, const void *a, const void *b,
+                                    const void *c, void *d, size_t n, uint32_t *flags_out,
+                                    uint32_t *bus_out);
+
+/* reference host/include/cft.h:1899-1964 (5.12 character conversions).
+ * cft_format_decimal_digits is a formula of the format and is computed:
+ * 1 + ceil(p * log10 2), 9, 17, 36 and 73; 0 for a format out of range. */
+CFT_API size_t cft_format_decimal_digits(cft_format fmt);
+CFT_API cft_status cft_from_decimal_char(cft_device *dev, cft_format fmt, cft_round rnd,
+                                         const char *const *in, void *d, size_t n,
+                                         size_t *bad_index, uint32_t *flags_out);
+CFT_API cft_status cft_to_decimal_char(cft_device *dev, cft_format fmt, cft_round rnd,
+                                       const void *a, size_t digits, char *out, size_t cap,
+                                       size_t *len, uint32_t *flags_out);
+CFT_API cft_status cft_from_hex_char(cft_device *dev, cft_format fmt, cft_round rnd,
+                                     const char *const *in, void *d, size_t n,
+                                     size_t *bad_index, uint32_t *flags_out);
+CFT_API cft_status cft_to_hex_char(cft_device *dev, cft_format fmt, const void *a, char *out,
+                                   size_t cap, size_t *len);
+
+/* reference host/include/cft.h:1996-2002 (9.7 NaN payloads) */
+CFT_API cft_status cft_get_payload(cft_device *dev, cft_format fmt, const void *a, void *d,
+                                   size_t n);
+CFT_API cft_status cft_set_payload(cft_device *dev, cft_format fmt, const void *a, void *d,
+                                   size_t n);
+CFT_API cft_status cft_set_payload_signaling(cft_device *dev, cft_format fmt, const void *a,
+                                             void *d, size_t n);
+
+/* reference host/include/cft.h:3360-3371 (9.6 magnitude forms) */
+#define CFT_OFFPATH_MAG(name) \
+    CFT_API cft_status name(cft_device *dev, cft_format fmt, const void *a, const void *b, \
+                            void *d, size_t n, uint32_t *flags_out);
+CFT_OFFPATH_MAG(cft_min_mag) CFT_OFFPATH_MAG(cft_max_mag)
+CFT_OFFPATH_MAG(cft_minnum_mag) CFT_OFFPATH_MAG(cft_maxnum_mag)
+#undef CFT_OFFPATH_MAG
+
+/* reference host/include/cft.h:3307-3309 (5.7.1).  All three are 0 here:
+ * this library carries a subset of 754's required operations (none of the
+ * character conversions, for one), so it claims no version. */
+CFT_API int cft_is754version1985(void);
+CFT_API int cft_is754version2008(void);
+CFT_API int cft_is754version2019(void);
+
 #ifdef __cplusplus
 }
 #endif
