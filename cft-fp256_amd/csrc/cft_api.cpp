@@ -12,6 +12,7 @@
 #include "kernels.h"
 
 #include <limits.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -366,8 +367,24 @@ cft_status bind_out(cft_device *dev, int role, void *p, size_t bytes, void **dpt
 
 // ---- kernel timer (include/cft_mi355x.h) -------------------------------------
 
+hipError_t plant_launch(cft_device *dev)
+{
+#ifdef CFT_PLANTS
+    if (dev->plant_fail_launch) {
+        dev->plant_fail_launch = 0;
+        return hipErrorLaunchFailure;
+    }
+#else
+    (void)dev;
+#endif
+    return hipSuccess;
+}
+
 hipError_t timer_begin(cft_device *dev)
 {
+    const hipError_t e = plant_launch(dev);
+    if (e != hipSuccess)
+        return e;
     return dev->timing ? hipEventRecord(dev->ev0, dev->stream) : hipSuccess;
 }
 
@@ -476,9 +493,25 @@ static cft_status open_impl(const char *artifact, int index, uint32_t depth, cft
                       count);
         return CFT_ERR_NO_DEVICE;
     }
+#ifdef CFT_PLANTS
+    // the negative-control build: CFT_PLANT names the one thing to break
+    int plant_fail = 0;
+    if (const char *pl = getenv("CFT_PLANT")) {
+        if (strcmp(pl, "fail-launch") == 0) {
+            plant_fail = 1;
+        } else if (*pl) {
+            cft_set_error("cft_open: CFT_PLANT=\"%s\" is not a plant of this build (the one "
+                          "plant is fail-launch)", pl);
+            return CFT_ERR_INVALID_ARGUMENT;
+        }
+    }
+#endif
     cft_device *D = new (std::nothrow) cft_device();
     if (!D)
         return CFT_ERR_OUT_OF_MEMORY;
+#ifdef CFT_PLANTS
+    D->plant_fail_launch = plant_fail;
+#endif
     D->hip_index = (int)num;
     D->scratch_depth = depth;
     hipDeviceProp_t prop;

@@ -84,11 +84,20 @@ struct cft_device {
     // the last multi-program launch's plan: waves a group, groups, LDS rows a
     // wave, tiles, groups a CU holds, planned makespan (bundles)
     uint64_t multi_plan[6] = {0, 0, 0, 0, 0, 0};
+#ifdef CFT_PLANTS
+    int plant_fail_launch = 0;     // CFT_PLANT=fail-launch: the next launch fails
+#endif
 };
+
+// Before a handle's compute launch.  hipSuccess, except in the negative-
+// control build (-DCFT_PLANTS, lib/libcft_plants.so), where a handle opened
+// with CFT_PLANT=fail-launch has its first launch refused here, before
+// anything is launched, so the failure path runs with no kernel involved.
+hipError_t plant_launch(cft_device *dev);
 
 // Bracket a compute-kernel launch with the handle's timer events (no-ops
 // unless cftx_kernel_timer enabled them); timer_end adds the elapsed time
-// once the stream has been synchronised.
+// once the stream has been synchronised.  timer_begin calls plant_launch.
 hipError_t timer_begin(cft_device *dev);
 hipError_t timer_end(cft_device *dev);
 void timer_collect(cft_device *dev);

@@ -243,8 +243,8 @@ hipError_t launch_elem(Worker &w)
     Batch &B = w.batch;
     cft_device *ex = w.dev;
     constexpr size_t area = (size_t)CFTK_FLAG_SLOTS * CFTK_SLOT_WORDS;
-    hipError_t e = hipSuccess;
-    if (w.flags_dirty) {
+    hipError_t e = plant_launch(ex);
+    if (e == hipSuccess && w.flags_dirty) {
         e = cftk_fill32(w.dflags, 0u, kMaxElemBatch * area, ex->stream);
         if (e == hipSuccess)
             w.flags_dirty = false;

@@ -377,17 +377,35 @@ static void unpack(const oc_fmt *f, const oc_big *x, unp *u)
 
 /* ---- rounding ------------------------------------------------------------ */
 
+/* Negative controls (tests/test_negative_controls.py).  A build with
+ * -DORACLE_PLANT=k breaks one rule on purpose so that the comparison with
+ * the reference-written golden vectors must fail: 1 RMM rounds as RNE,
+ * 2 UNDERFLOW is dropped for tiny inexact results, 3 RNE ties go to odd,
+ * 4 tininess is decided before rounding.  The default build (and every
+ * build the checks use) has none of them. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 /* host/src/softfloat.c:120-134: does a mode increment, given the sign,
  * the first discarded bit, whether anything below it is set, and the
  * kept lsb. */
 static int round_incr(int rnd, int sign, int guard, int sticky, int lsb)
 {
     switch (rnd) {
+#if ORACLE_PLANT == 3
+    case RNE: return guard && (sticky || !lsb);
+#else
     case RNE: return guard && (sticky || lsb);
+#endif
     case RTZ: return 0;
     case RDN: return sign && (guard || sticky);
     case RUP: return !sign && (guard || sticky);
+#if ORACLE_PLANT == 1
+    default:  return guard && (sticky || lsb);
+#else
     default:  return guard;                       /* RMM */
+#endif
     }
 }
 
@@ -437,7 +455,11 @@ static void round_pack(const oc_fmt *f, int sign, const oc_big *m, int e,
 
     e_norm = e + L - 1;
     round_to(m, e, e_norm - (p - 1), sign, rnd, &ku, &dummy);
+#if ORACLE_PLANT == 4
+    tiny = e_norm < f->emin;
+#else
     tiny = (e_norm - (p - 1) + bbitlen(&ku) - 1) < f->emin;
+#endif
 
     q = (e_norm > f->emin ? e_norm : f->emin) - (p - 1);
     round_to(m, e, q, sign, rnd, &k, &inexact);
@@ -457,7 +479,7 @@ static void round_pack(const oc_fmt *f, int sign, const oc_big *m, int e,
         *flags = fl | OC_FLAG_OVERFLOW | OC_FLAG_INEXACT;
         return;
     }
-    if (tiny && inexact)
+    if (tiny && inexact && ORACLE_PLANT != 2)
         fl |= OC_FLAG_UNDERFLOW;
     if (e_res < f->emin) {
         /* subnormal: k's lsb weighs 2^(emin - p + 1), so k is the field */
