@@ -23,10 +23,24 @@ Beside it, in the same JSON line:
               p50/p99 latency, throughput, every job checked against the job
               run on its own, the plain one-job-a-launch form on a prefix;
   cpu_baseline  the reference's CPU FMA on this host's cores, same inputs,
-              compared bit for bit;
+              compared bit for bit, all threads and one core, with the host's
+              CPU model and counts;
   roofline    the FMA kernel against 8 TB/s, and against the rate a library
               ternary elementwise kernel (torch.addcmul) streams over the same
-              bytes (stream_GBs, frac_of_stream).
+              bytes (stream_GBs, frac_of_stream); beside it the VALU figures:
+              the issue rates of the integer instructions measured live
+              (lib/libcftprobe.so), the FMA's VALU issue and its algorithmic
+              VALU fraction (64 multiply-accumulates an element at their
+              measured issue cost).
+
+Checks in the same run, each reported with its size: add and mul over all
+5 x 2^24 results against the reference's CPU library, div and sqrt on every
+16th element of each mode; the Lorenz-63 ensemble's final states of its 32
+longest and 224 rank-stratified trajectories against the reference's chain
+of segments; one Lorenz-96 instance a ring size in each format; every
+served elementwise job's bits against the job run alone, 32 elements of each
+(and every element of jobs up to 20,000) against the reference, and a
+stratified sample of the served ODE lanes against the reference.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --copy-calibration BYTES     (a plain device copy, for
@@ -36,6 +50,7 @@ Beside it, in the same JSON line:
 import argparse
 import ctypes
 import json
+from concurrent.futures import ThreadPoolExecutor
 import os
 import sys
 import threading
@@ -54,6 +69,15 @@ FMA_BYTES = 128          # 3 x 32 B in, 32 B out per element
 # plain 256 MiB copy (profiles/fma_fp256_elementwise.md)
 FMA_TRAFFIC_PER_ELEM = (393543.0 + 131129.0) * 1024 / 4194304
 BYTES = {"add": 96, "mul": 96, "fma": 128, "div": 96, "sqrt": 64}
+# VALU: 256 CUs x 4 SIMDs x 2.4 GHz, a wave64 instruction every 4 cycles
+VALU_PEAK_WAVE_INSNS = 256 * 4 * 2.4e9 / 4
+VALU_PEAK_LANE_OPS = VALU_PEAK_WAVE_INSNS * 64
+# VALU instructions a lane issues for one binary256 FMA element of this
+# workload (rocprofv3 SQ_INSTS_VALU / SQ_WAVES at 2^22 elements,
+# profiles/elementwise_ops_valu.md); the exact product's 32x32->64
+# multiply-accumulates an element (8 x 8 limbs)
+FMA_VALU_PER_ELEM = 854.6
+FMA_MACS = 64
 
 
 def log(*a):
@@ -77,42 +101,137 @@ def parse():
     return ap.parse_args()
 
 
-def ref_elem(a, b, c, n, fmt, op, threads):
-    """One elementwise op through the reference's CPU library (oracle/_ref),
-    one software device per thread: (d, flags, seconds) or None."""
-    path = os.path.join(HERE, "oracle", "_ref", "libcft_ref.so")
+def host_info():
+    """nproc, the CPUs this process may use, physical cores and the CPU
+    model, from the kernel's own tables."""
+    model, phys = None, set()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            pid = None
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    phys.add((pid, v))
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "cpus_available": avail,
+            "physical_cores": len(phys) or None, "cpu_model": model}
+
+
+def cpu_threads(args, info):
+    """Threads for the CPU legs: --cpu-threads, else one per physical core
+    the process may use (OMP_NUM_THREADS, where set, is the host's share)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    t = min(info["cpus_available"], info["physical_cores"] or info["cpus_available"])
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(t, omp) if omp > 0 else t)
+
+
+class RefCPU:
+    """The reference's own C library (oracle/_ref/libcft_ref.so, compiled
+    from its sources by oracle/Makefile): its software device, one per
+    thread over contiguous slices.  `ok` is False where it is not built."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "oracle", "_ref", "libcft_ref.so")
+        self.ok = os.path.exists(path)
+        if not self.ok:
+            return
+        L = ctypes.CDLL(path)
+        V, U32P = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)
+        L.cft_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(V)]
+        L.cft_close.argtypes = [V]
+        L.cft_run.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
+                              ctypes.c_size_t, U32P, U32P]
+        L.cft_div.argtypes = [V, ctypes.c_int, ctypes.c_int, V, V, V, ctypes.c_size_t, U32P,
+                              U32P]
+        L.cft_sqrt.argtypes = [V, ctypes.c_int, ctypes.c_int, V, V, ctypes.c_size_t, U32P, U32P]
+        self.L = L
+
+    def elem(self, name, fmt, rnd, a, b, c, n, threads):
+        """d = op(a, b, c) for name in fma/add/mul/div/sqrt (ADD reads a and
+        c, MUL a and b); returns (d, flags ORed, seconds)."""
+        L, V = self.L, ctypes.c_void_p
+        esz = {0: 4, 1: 8, 2: 16, 3: 32}[fmt]
+        d = np.empty(n * esz, dtype=np.uint8)
+        flags = [0] * threads
+        bad = []
+        p = lambda x, off: None if x is None else x.ctypes.data + off
+
+        def work(t):
+            lo, hi = n * t // threads, n * (t + 1) // threads
+            if hi <= lo:
+                return
+            dev = V()
+            L.cft_open(None, 0, ctypes.byref(dev))
+            f, o = ctypes.c_uint32(), esz * lo
+            if name == "fma":
+                st = L.cft_run(dev, 0, fmt, rnd, p(a, o), p(b, o), p(c, o), p(d, o), hi - lo,
+                               ctypes.byref(f), None)
+            elif name == "add":
+                st = L.cft_run(dev, 1, fmt, rnd, p(a, o), None, p(c, o), p(d, o), hi - lo,
+                               ctypes.byref(f), None)
+            elif name == "mul":
+                st = L.cft_run(dev, 3, fmt, rnd, p(a, o), p(b, o), None, p(d, o), hi - lo,
+                               ctypes.byref(f), None)
+            elif name == "div":
+                st = L.cft_div(dev, fmt, rnd, p(a, o), p(b, o), p(d, o), hi - lo,
+                               ctypes.byref(f), None)
+            else:
+                st = L.cft_sqrt(dev, fmt, rnd, p(a, o), p(d, o), hi - lo, ctypes.byref(f), None)
+            L.cft_close(dev)
+            if st != 0:
+                bad.append(st)
+            flags[t] = f.value
+        t0 = time.perf_counter()
+        ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        if bad:
+            raise RuntimeError(f"the reference refused a call: {bad[0]}")
+        fl = 0
+        for f in flags:
+            fl |= f
+        return d, fl, time.perf_counter() - t0
+
+
+def issue_rates():
+    """Wave instructions a second the GPU issues of v_add_u32, v_mad_u64_u32,
+    v_mul_lo_u32, v_mul_hi_u32 and a v_add_co_u32 + v_addc_co_u32 pair
+    (lib/libcftprobe.so: 8 independent chains a lane, 8 waves a SIMD), and
+    each relative to v_add_u32."""
+    path = os.path.join(HERE, "cft-fp256_amd", "lib", "libcftprobe.so")
     if not os.path.exists(path):
         return None
-    lib = ctypes.CDLL(path)
-    V, U32P = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)
-    lib.cft_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(V)]
-    lib.cft_run.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
-                            ctypes.c_size_t, U32P, U32P]
-    lib.cft_close.argtypes = [V]
-    esz = {2: 16, 3: 32}[fmt]
-    d = np.empty(n * esz, dtype=np.uint8)
-    flags = [0] * threads
-
-    def work(t):
-        lo, hi = n * t // threads, n * (t + 1) // threads
-        dev = V()
-        lib.cft_open(None, 0, ctypes.byref(dev))
-        f = ctypes.c_uint32()
-        lib.cft_run(dev, op, fmt, 0, a.ctypes.data + esz * lo, b.ctypes.data + esz * lo,
-                    c.ctypes.data + esz * lo, d.ctypes.data + esz * lo, hi - lo,
-                    ctypes.byref(f), None)
-        lib.cft_close(dev)
-        flags[t] = f.value
-    t0 = time.perf_counter()
-    ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
-    for th in ths:
-        th.start()
-    for th in ths:
-        th.join()
-    fl = 0
-    for f in flags:
-        fl |= f
-    return d, fl, time.perf_counter() - t0
+    P = ctypes.CDLL(path)
+    P.cftp_issue_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for kind, name in enumerate(("v_add_u32", "v_mad_u64_u32", "v_mul_lo_u32",
+                                 "v_mul_hi_u32", "v_add_co_u32+v_addc_co_u32")):
+        ms, wi = ctypes.c_double(), ctypes.c_double()
+        st = P.cftp_issue_rate(kind, 2048, 16384, 5, ctypes.byref(ms), ctypes.byref(wi))
+        if st != 0:
+            return None
+        out[name] = wi.value / (ms.value * 1e-3)
+    rel = {k: round(v / out["v_add_u32"], 4) for k, v in out.items()}
+    return {"wave_insns_per_s": {k: round(v / 1e9, 1) for k, v in out.items()},
+            "unit": "G wave instructions/s", "relative_to_v_add_u32": rel,
+            "probe": "lib/libcftprobe.so cftp_issue_rate: 2,048 workgroups x 256 lanes, "
+                     "16,384 rounds of 8 independent chains, mean of 5 launches"}
 
 
 def copy_calibration(nbytes):
@@ -155,11 +274,13 @@ def stream_probe(n_elem, elem_bytes=32, reps=5):
         return None
 
 
-def ref_program_chain(img, bank, state, steps, fmt, threads):
+def ref_program_chain(img, bank, state, steps, fmt, threads, depth=2048):
     """The reference's CPU program runs (oracle/_ref, its own C library) on a
     chain of segments per lane - full segments at the image's trip count,
     then the remainder - one software device per thread over slices of
-    lanes.  Returns (final states, flags, status, seconds) or None."""
+    lanes, each opened at scratch depth `depth` (cft_open_ex; Lorenz-96 at
+    ring K needs 5K slots).  Returns (final states, flags, status, seconds)
+    or None."""
     import ctypes as C
     sys.path.insert(0, os.path.join(HERE, "cft-fp256_amd"))
     from cftamd import programs as P
@@ -181,7 +302,10 @@ def ref_program_chain(img, bank, state, steps, fmt, threads):
                     ("idx_scratch_src", C.c_size_t), ("lane_mask", V),
                     ("lane_mask_bytes", C.c_size_t), ("lane_flags", V),
                     ("lane_flags_bytes", C.c_size_t)]
-    lib.cft_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(V)]
+    class OpenArgs(C.Structure):
+        _fields_ = [("struct_size", C.c_size_t), ("artifact", C.c_char_p),
+                    ("index", C.c_int), ("scratch_depth", C.c_uint32)]
+    lib.cft_open_ex.argtypes = [C.POINTER(OpenArgs), C.POINTER(V)]
     lib.cft_close.argtypes = [V]
     lib.cft_program_load.argtypes = [V, V, C.c_size_t, C.POINTER(V)]
     lib.cft_program_free.argtypes = [V]
@@ -196,7 +320,10 @@ def ref_program_chain(img, bank, state, steps, fmt, threads):
 
     def work(lo, hi):
         dev = V()
-        lib.cft_open(None, 0, C.byref(dev))
+        oa = OpenArgs(C.sizeof(OpenArgs), None, 0, depth)
+        if lib.cft_open_ex(C.byref(oa), C.byref(dev)) != 0:
+            res["bad"] += 1
+            return
         progs = {}
         left = steps[lo:hi].astype(np.int64).copy()
         fl = stt = 0
@@ -249,7 +376,7 @@ def ref_program_chain(img, bank, state, steps, fmt, threads):
     return st.reshape(-1), res["flags"], res["status"], secs
 
 
-def rk4_section(dev, lib, args, threads):
+def rk4_section(dev, lib, args, threads, ref):
     """Lorenz-63 RK4 binary256 ensemble: persistent batched runner on the whole
     ensemble, per-op form against it on a 1/100 sample, the reference's CPU
     runs on a bounded sample."""
@@ -314,22 +441,35 @@ def rk4_section(dev, lib, args, threads):
     for b in (ta, tb):
         b.free()
 
-    # the reference's CPU runs on 1,024 trajectories capped at 2,000 steps
+    # the full run's final states against the reference's CPU chain of
+    # segments: the 32 longest trajectories and 224 more at evenly spaced
+    # ranks of the step counts, each at its full length
     if not args.skip_cpu:
-        ids = np.linspace(0, steps.size - 1, 1024).astype(np.int64)
-        cs = np.minimum(steps[ids], 2000).astype(np.uint32)
-        cstate = state.reshape(steps.size, 96)[ids].reshape(-1).copy()
-        r = ref_program_chain(img, bank, cstate, cs, 3, threads)
+        order = np.argsort(steps, kind="stable")[::-1]
+        ranks = np.linspace(32, steps.size - 1, 224).astype(np.int64)
+        subset = np.concatenate([order[:32], order[ranks]])
+        by_len = subset[np.argsort(steps[subset], kind="stable")[::-1]]
+        # thread t takes lanes by_len[t::threads] (contiguous in this order)
+        subset = np.concatenate([by_len[t::threads] for t in range(threads)])
+        cs = steps[subset].astype(np.uint32)
+        st3 = state.reshape(steps.size, 96)
+        r = ref_program_chain(img, bank, st3[subset].reshape(-1).copy(), cs, 3, threads)
         if r is not None:
-            g = cstate.copy()
-            prog.run_batch(g, cs, bank=bank)
+            got = full.reshape(steps.size, 96)[subset]
+            eq = r[0].reshape(-1, 96) == got
+            lanes_bad = int(np.count_nonzero(~eq.all(axis=1)))
             out["lorenz63_fp256_cpu"] = {
                 "kind": "reference", "cores": threads, "value": round(int(cs.sum()) / r[3], 1),
-                "unit": "steps/s", "sample": f"1,024 trajectories of the ensemble (evenly "
-                f"spaced), steps capped at 2,000 ({int(cs.sum())} steps), {threads} threads, "
-                f"{r[3]:.1f} s wall", "bit_equal_gpu": bool(np.array_equal(r[0], g)) and
-                (r[1], r[2]) == (fl, st)}
-            log(f"bench: rk4 CPU {int(cs.sum()) / r[3]:.0f} steps/s on {threads} threads")
+                "unit": "steps/s",
+                "sample": f"the full run's final states of its 32 longest trajectories and 224 "
+                          f"at evenly spaced step ranks, each at its full length ({int(cs.sum())} "
+                          f"steps, {int(cs.min())}..{int(cs.max())} a trajectory), the "
+                          f"reference's chain of 100-step segments, {threads} threads, "
+                          f"{r[3]:.1f} s wall",
+                "trajectories_checked": int(subset.size), "lanes_differing": lanes_bad,
+                "bit_equal_gpu": lanes_bad == 0 and (r[1] | fl) == fl and r[2] == st}
+            log(f"bench: rk4 CPU {int(cs.sum()) / r[3]:.0f} steps/s on {threads} threads; "
+                f"{lanes_bad} of {subset.size} full-length trajectories differ")
     for b in (sbuf, bbuf):
         b.free()
     prog.free()
@@ -354,6 +494,34 @@ def rk4_section(dev, lib, args, threads):
             fl, _s, widths = run_multi(d2, parts, **knobs)
             secs = time.perf_counter() - t0
             plan = last_plan(d2)
+            # one instance a ring size - its fewest steps, at full length -
+            # against the reference's CPU chain of segments, a thread each
+            cpu = None
+            if not args.skip_cpu and ref.ok:
+                jobs = []
+                for g, q in zip(groups, parts):
+                    j = int(np.argmin(g[5]))
+                    row = g[4].nbytes // g[5].size
+                    jobs.append((g, g[4][j * row:(j + 1) * row].copy(), g[5][j:j + 1].copy(),
+                                 q["state"][j * row:(j + 1) * row].copy()))
+                t0 = time.perf_counter()
+                with ThreadPoolExecutor(threads) as ex:
+                    rr = list(ex.map(lambda jb: ref_program_chain(jb[0][1], jb[0][2], jb[1],
+                                                                  jb[2], fmt, 1), jobs))
+                tc = time.perf_counter() - t0
+                bad = sum(1 for jb, r in zip(jobs, rr)
+                          if r is None or not np.array_equal(r[0], jb[3]) or (r[1] | fl) != fl)
+                csteps = sum(int(jb[2][0]) for jb in jobs)
+                ccomp = sum(int(jb[2][0]) * jb[0][0] for jb in jobs)
+                cpu = {"kind": "reference", "cores": threads, "value": round(csteps / tc, 1),
+                       "unit": "steps/s", "component_steps_per_s": round(ccomp / tc, 1),
+                       "sample": f"one instance of each of the {len(jobs)} ring sizes (its "
+                                 f"fewest steps, {csteps} steps in all) at full length from the "
+                                 f"one-launch run, the reference's chain of 20-step segments, "
+                                 f"{threads} threads, {tc:.1f} s wall",
+                       "instances_checked": len(jobs), "instances_differing": bad,
+                       "bit_equal_gpu": bad == 0}
+                log(f"bench: {name} CPU {csteps / tc:.0f} steps/s; {bad} of {len(jobs)} differ")
             # on the sample: the same one launch against one batched launch a
             # ring size, bit for bit
             sp = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=s2)
@@ -372,6 +540,24 @@ def rk4_section(dev, lib, args, threads):
             for o, q in zip(outs, sp):
                 eq &= bool(np.array_equal(o, q["state"]))
             stot = int(sum(int(x.sum()) for x in samp))
+            # the plain form - one kernel launch an instruction, a batch a
+            # ring size - beside the one launch, on steps // 2000 (at least 1)
+            samp2 = [np.maximum(1, g[5] // 2000).astype(np.uint32) for g in groups]
+            sq = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=s2)
+                  for p, g, s2 in zip(progs, groups, samp2)]
+            t0 = time.perf_counter()
+            run_multi(d2, sq, **knobs)
+            t_one = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            nl, eq2 = 0, True
+            for p, g, s2, q in zip(progs, groups, samp2, sq):
+                st2 = g[4].copy()
+                p.run_batch(st2, s2, bank=g[2], mode=L.BATCH_PER_OP)
+                nl += int(lib.cftx_batch_launches(d2.handle))
+                eq2 &= bool(np.array_equal(st2, q["state"]))
+            t_op = time.perf_counter() - t0
+            log(f"bench: {name} per-op form {t_op:.2f} s ({nl} launches) vs one launch "
+                f"{t_one:.3f} s on steps // 2000")
             for p in progs:
                 p.free()
             out[name] = {"instances": 8192, "ring_sizes": len(groups), "total_steps": tot,
@@ -384,16 +570,108 @@ def rk4_section(dev, lib, args, threads):
                          "sample": {"steps": "steps // 100 (at least 1)", "total_steps": stot,
                                     "one_launch_s": round(t_multi, 4),
                                     "launch_a_ring_s": round(t_perk, 4),
-                                    "speedup": round(t_perk / t_multi, 2), "bit_equal": eq}}
+                                    "speedup": round(t_perk / t_multi, 2), "bit_equal": eq},
+                         "per_op_sample": {"steps": "steps // 2000 (at least 1)",
+                                           "total_steps": int(sum(int(x.sum()) for x in samp2)),
+                                           "one_launch_s": round(t_one, 4),
+                                           "per_op_s": round(t_op, 4), "per_op_launches": nl,
+                                           "speedup_vs_per_op": round(t_op / t_one, 2),
+                                           "bit_equal": eq2},
+                         "cpu": cpu}
             log(f"bench: {name} {tot / secs / 1e6:.2f} M steps/s ({secs:.2f} s, one launch); "
                 f"sample one launch {t_multi:.3f} s vs a launch a ring {t_perk:.3f} s")
     return out
 
 
-def serve_section(args):
+def serve_oracle(pools, reqs, res, samples, threads, ref):
+    """The served stream against the reference's CPU library: 32 elements of
+    every elementwise job (its first, its last and 30 seeded others), every
+    element and the flags of every job of at most 20,000 elements, and lane
+    0 of a stratified sample of the ODE jobs (8 Lorenz-63 and 8 Henon-Heiles
+    jobs at evenly spaced step ranks, the 6 Lorenz-96 jobs of least work) at
+    full length."""
+    from cftamd import serve as S
+    if not ref.ok:
+        return None
+    names = {0: "fma", 1: "div", 2: "sqrt"}
+
+    def operands(kind, fmt, pick):
+        """a, b, c of a job kind from the pools, at element rows `pick`."""
+        esz = 16 if fmt == 2 else 32
+        ops = pools.ops[fmt]
+        take = lambda k: ops[k].host.reshape(-1, esz)[pick].reshape(-1).copy()
+        return (take(3 if kind == 2 else 0), take(1) if kind in (0, 1) else None,
+                take(2) if kind == 0 else None)
+
+    t0 = time.perf_counter()
+    groups = {}
+    for r, _fl, idx, got, _whole in samples:
+        groups.setdefault((r["kind"], r["fmt"]), []).append((r, idx, got))
+    el_checked = el_bad = 0
+    for (kind, fmt), items in sorted(groups.items()):
+        esz = 16 if fmt == 2 else 32
+        pick = np.concatenate([r["off"] + idx for r, idx, _g in items])
+        a, b, c = operands(kind, fmt, pick)
+        want, _f, _t = ref.elem(names[kind], fmt, 0, a, b, c, pick.size, threads)
+        got = np.concatenate([g.reshape(-1) for _r, _i, g in items])
+        diff = (want.reshape(-1, esz) != got.reshape(-1, esz)).any(axis=1)
+        el_checked += int(diff.size)
+        el_bad += int(diff.sum())
+
+    def whole_job(item):
+        r, fl, whole = item
+        fmt, n = r["fmt"], r["n"]
+        a, b, c = operands(r["kind"], fmt, np.arange(r["off"], r["off"] + n))
+        want, wfl, _t = ref.elem(names[r["kind"]], fmt, 0, a, b, c, n, 1)
+        return bool(np.array_equal(want, whole)) and wfl == fl
+
+    small = [(r, fl, whole) for r, fl, _i, _g, whole in samples if whole is not None]
+    with ThreadPoolExecutor(threads) as ex:
+        small_ok = list(ex.map(whole_job, small))
+
+    picks = []
+    for kind in (3, 4, 5):
+        js = [(r, q) for r, q in zip(reqs, res) if r["kind"] == kind]
+        if not js:
+            continue
+        if kind == 4:
+            js.sort(key=lambda rq: rq[0]["k"] * int(np.asarray(rq[0]["steps"])[0]))
+            picks += js[:6]
+        else:
+            js.sort(key=lambda rq: int(np.asarray(rq[0]["steps"])[0]))
+            at = sorted(set(np.linspace(0, len(js) - 1, min(8, len(js))).astype(int).tolist()))
+            picks += [js[i] for i in at]
+
+    def lane0(rq):
+        r, q = rq
+        lanes = np.asarray(r["steps"]).size
+        row = r["state_bytes"] // lanes
+        lo = r["state_off"]
+        out = ref_program_chain(pools.image_of(r), pools.bank_of(r),
+                                pools.state0[lo:lo + row].copy(),
+                                np.asarray(r["steps"], dtype=np.uint32)[:1].copy(), 3, 1)
+        return (out is not None and bool(np.array_equal(out[0], pools.states.host[lo:lo + row]))
+                and (out[1] | q.flags) == q.flags, int(np.asarray(r["steps"])[0]))
+
+    with ThreadPoolExecutor(threads) as ex:
+        lanes = list(ex.map(lane0, picks))
+    secs = time.perf_counter() - t0
+    return {"kind": "reference", "threads": threads, "seconds": round(secs, 1),
+            "elementwise_sampled_elements": el_checked, "elementwise_sampled_differing": el_bad,
+            "elementwise_jobs_sampled": len(samples),
+            "whole_jobs_checked": len(small), "whole_jobs_differing": small_ok.count(False),
+            "whole_job_limit_elements": 20000,
+            "ode_lanes_checked": len(lanes), "ode_lanes_differing": sum(not ok for ok, _s in lanes),
+            "ode_lane_steps": [s for _ok, s in lanes],
+            "bit_exact": el_bad == 0 and all(small_ok) and all(ok for ok, _s in lanes)}
+
+
+def serve_section(args, threads, ref):
     """The serving loop over a Poisson stream at 2,000 requests/s for one
     second (cftamd/serve.py states the mix and sizes): p50/p99 latency and
-    throughput, every job checked against the same job run on its own, and
+    throughput, every job checked against the same job run on its own (every
+    elementwise job's output bits in a window of its own), a sample checked
+    against the reference's CPU library (serve_oracle), and
     the plain form - one worker, one job a launch, program jobs run whole -
     on the stream's first 20 ms beside the batched loop on the same
     requests."""
@@ -402,7 +680,8 @@ def serve_section(args):
     out = {}
     with Device("hip:0", scratch_depth=2048) as dev:
         reqs = S.requests(seed=args.seed, rate=2000, duration=1.0)
-        pools = S.Pools(dev, reqs, seed=args.seed)
+        # every elementwise job writes a window of its own (device memory)
+        pools = S.Pools(dev, reqs, seed=args.seed, device_out=True)
         srv = S.Server(dev, workers=4)
         res, wall = S.serve(srv, pools, reqs)
         stats = srv.stats()
@@ -410,8 +689,22 @@ def serve_section(args):
         summ = S.summarize(reqs, res, wall)
         log(f"bench: serve p50 {summ['p50_ms']} ms p99 {summ['p99_ms']} ms over "
             f"{len(reqs)} requests ({wall:.2f} s)")
-        v = S.verify(dev, pools, reqs, res, alone_every=100)
+        samples = []
+
+        def hook(r, q, served):
+            esz, n = (16 if r["fmt"] == 2 else 32), r["n"]
+            rng = np.random.default_rng(r["off"] * 7919 + n)
+            idx = np.unique(np.concatenate([[0, n - 1], rng.integers(0, n, 30)]))
+            samples.append((r, q.flags, idx, served.reshape(n, esz)[idx].copy(),
+                            served.copy() if n <= 20000 else None))
+
+        v = S.verify(dev, pools, reqs, res, alone_every=100, hook=hook)
+        n_elem = sum(1 for r in reqs if r["kind"] <= 2)
+        v["elementwise_jobs"] = n_elem
         log(f"bench: serve verification {v}")
+        orc = None if args.skip_cpu else serve_oracle(pools, reqs, res, samples, threads, ref)
+        log(f"bench: serve against the reference {orc}")
+        del samples
         pre = [r for r in reqs if r["t"] < 0.02]
         pools.states.host[: pools.state0.nbytes] = pools.state0
         pools.states.to_device()
@@ -431,6 +724,7 @@ def serve_section(args):
                "served_req_s": summ["throughput_req_s"], "by_kind": summ["by_kind"],
                "elements": summ["elements"], "ode_steps": summ["ode_steps"],
                "statuses_ok": summ["statuses_ok"], "server": stats, "verification": v,
+               "vs_reference": orc,
                "plain_form_prefix": {"requests": len(pre), "p50_ms": plain["p50_ms"],
                                      "p99_ms": plain["p99_ms"], "wall_s": plain["wall_s"],
                                      "batched_p50_ms": same["p50_ms"],
@@ -438,64 +732,98 @@ def serve_section(args):
     return out
 
 
-def cpu_baseline(a, b, c, gpu_out, n, threads):
+def cpu_baseline(a, b, c, gpu_out, n, threads, ref, info):
     """The reference's CPU path (its own C library, compiled from its sources
     into oracle/_ref by oracle/Makefile) on the same operands: one software
-    device per thread over contiguous slices, every mode.  Falls back to the
-    oracle restatement when the reference library is not present."""
-    ref_path = os.path.join(HERE, "oracle", "_ref", "libcft_ref.so")
+    device per thread over contiguous slices, every mode, every result
+    compared; then one thread on the first 2^18 elements (RNE) for the
+    single-core rate.  Falls back to the oracle restatement when the
+    reference library is not present."""
     port_path = os.path.join(HERE, "oracle", "_build", "libcftoracle.so")
     V, U32P = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)
-    if os.path.exists(ref_path):
+    if ref.ok:
         kind = "reference"
-        lib = ctypes.CDLL(ref_path)
-        lib.cft_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(V)]
-        lib.cft_run.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
-                                ctypes.c_size_t, U32P, U32P]
-        lib.cft_close.argtypes = [V]
 
-        def run_slice(rnd, lo, hi, out):
-            dev = V()
-            assert lib.cft_open(None, 0, ctypes.byref(dev)) == 0
-            f = ctypes.c_uint32()
-            st = lib.cft_run(dev, 0, 3, rnd, a.ctypes.data + 32 * lo, b.ctypes.data + 32 * lo,
-                             c.ctypes.data + 32 * lo, out.ctypes.data + 32 * lo, hi - lo,
-                             ctypes.byref(f), None)
-            lib.cft_close(dev)
-            assert st == 0
+        def run_all(rnd, nn, th):
+            return ref.elem("fma", 3, rnd, a, b, c, nn, th)[0]
     elif os.path.exists(port_path):
         kind = "port"
         lib = ctypes.CDLL(port_path)
         lib.oc_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
                                ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int, U32P]
 
-        def run_slice(rnd, lo, hi, out):
+        def run_all(rnd, nn, th):
+            out = np.empty(nn * 32, dtype=np.uint8)
             f = ctypes.c_uint32()
-            lib.oc_run(3, 0, rnd, a.ctypes.data + 32 * lo, b.ctypes.data + 32 * lo,
-                       c.ctypes.data + 32 * lo, out.ctypes.data + 32 * lo, hi - lo, 0, 1,
-                       ctypes.byref(f))
+            lib.oc_run(3, 0, rnd, a.ctypes.data, b.ctypes.data, c.ctypes.data,
+                       out.ctypes.data, nn, 0, th, ctypes.byref(f))
+            return out
     else:
         return None, None
-    out = np.empty(n * 32, dtype=np.uint8)
     mismatches = 0
     t0 = time.perf_counter()
     for rnd in range(5):
-        ths = []
-        for t in range(threads):
-            lo, hi = n * t // threads, n * (t + 1) // threads
-            th = threading.Thread(target=run_slice, args=(rnd, lo, hi, out))
-            th.start()
-            ths.append(th)
-        for th in ths:
-            th.join()
+        out = run_all(rnd, n, threads)
         mismatches += int(np.count_nonzero(
             (out.reshape(-1, 32) != gpu_out[rnd][: n * 32].reshape(-1, 32)).any(axis=1)))
     secs = time.perf_counter() - t0
+    n1 = min(n, 1 << 18)
+    t1 = time.perf_counter()
+    one = run_all(0, n1, 1)
+    s1 = time.perf_counter() - t1
+    mismatches += int(np.count_nonzero(
+        (one.reshape(-1, 32) != gpu_out[0][: n1 * 32].reshape(-1, 32)).any(axis=1)))
     return {"value": round(5 * n / secs / 1e9, 6), "unit": "Gelem/s", "cores": threads,
             "kind": kind,
             "sample": f"binary256 FMA, the first {n} elements of the benchmark's operands, "
                       f"all five rounding modes ({5 * n} elements), {threads} threads, one "
-                      f"software device per thread, {secs:.1f} s wall"}, mismatches
+                      f"software device per thread, {secs:.1f} s wall; single core: the "
+                      f"first {n1} elements, RNE, {s1:.2f} s",
+            "single_core": {"value": round(n1 / s1 / 1e9, 6), "unit": "Gelem/s",
+                            "ns_per_elem": round(s1 / n1 * 1e9, 1)},
+            "all_cores_ns_per_elem": round(secs / (5 * n) * 1e9 * threads, 1),
+            "nproc": info["nproc"], "cpus_available": info["cpus_available"],
+            "physical_cores": info["physical_cores"], "cpu_model": info["cpu_model"]}, \
+        mismatches
+
+
+def check_op(lib, dev, bufs, n, name, rnd, ref, threads, sample):
+    """One binary256 op over the resident operands against the reference's
+    CPU library: every element (add, mul) or every 16th (div, sqrt), and for
+    the sample the GPU's own call over the gathered elements beside it, so
+    flags are compared exactly.  Returns (elements checked, differing)."""
+    fl = ctypes.c_uint32()
+    a, b, c, sa = bufs[0], bufs[1], bufs[2], bufs[4]
+    if name == "add":
+        st = lib.cft_run(dev.handle, 1, 3, rnd, a.data_ptr, None, c.data_ptr, bufs[3].data_ptr,
+                         n, ctypes.byref(fl), None)
+    elif name == "mul":
+        st = lib.cft_run(dev.handle, 3, 3, rnd, a.data_ptr, b.data_ptr, None, bufs[3].data_ptr,
+                         n, ctypes.byref(fl), None)
+    elif name == "div":
+        st = lib.cft_div(dev.handle, 3, rnd, a.data_ptr, b.data_ptr, bufs[3].data_ptr, n,
+                         ctypes.byref(fl), None)
+    else:
+        st = lib.cft_sqrt(dev.handle, 3, rnd, sa.data_ptr, bufs[3].data_ptr, n,
+                          ctypes.byref(fl), None)
+    assert st == 0, lib.cft_last_error()
+    bufs[3].from_device()
+    got = bufs[3].host.reshape(n, 32)
+    if not sample:
+        want, wfl, _t = ref.elem(name, 3, rnd, a.host, b.host, c.host, n, threads)
+        bad = int(np.count_nonzero((want.reshape(n, 32) != got).any(axis=1)))
+        return n, bad + (0 if wfl == fl.value else 1)
+    pick = np.arange(0, n, 16)
+    take = lambda x: x.host.reshape(n, 32)[pick].reshape(-1).copy()
+    xa, xb = (take(sa) if name == "sqrt" else take(a)), take(b)
+    want, wfl, _t = ref.elem(name, 3, rnd, xa, xb, None, pick.size, threads)
+    bad = int(np.count_nonzero((want.reshape(-1, 32) != got[pick]).any(axis=1)))
+    if name == "div":
+        g2, gfl = dev.div(3, rnd, xa, xb)
+    else:
+        g2, gfl = dev.sqrt(3, rnd, xa)
+    bad += int(np.count_nonzero((want.reshape(-1, 32) != g2.reshape(-1, 32)).any(axis=1)))
+    return pick.size, bad + (0 if wfl == gfl else 1)
 
 
 def main():
@@ -516,14 +844,22 @@ def main():
 
     from cftamd import Device, workload, _lib
     lib = _lib.lib()
+    info = host_info()
+    threads = cpu_threads(args, info)
+    ref = RefCPU()
+    log(f"bench: host {info}, {threads} CPU threads, reference library "
+        f"{'present' if ref.ok else 'absent'}")
     n = args.n
     log(f"bench: generating {n} binary256 operand triples (seed {args.seed})")
     t0 = time.perf_counter()
     dev = Device("hip:0")
-    bufs = [dev.alloc(n * 32) for _ in range(4)]
+    bufs = [dev.alloc(n * 32) for _ in range(5)]
     workload.elementwise(2, n, seed=args.seed, threads=16,
                          out=(bufs[0].host, bufs[1].host, bufs[2].host))
-    for bf in bufs[:3]:
+    # sqrt reads |a|
+    bufs[4].host[:] = bufs[0].host
+    bufs[4].host.reshape(n, 32)[:, 31] &= 0x7F
+    for bf in bufs[:3] + bufs[4:]:
         bf.to_device()
     log(f"bench: operands resident in HBM ({time.perf_counter() - t0:.1f} s)")
 
@@ -555,6 +891,8 @@ def main():
     achieved = FMA_BYTES * n / kavg_s / 1e9
     log(f"bench: FMA {value:.3f} Gelem/s call-to-call, kernel {kavg_s * 1e3:.3f} ms "
         f"per launch ({n / kavg_s / 1e9:.3f} Gelem/s in-kernel)")
+    rates = issue_rates()
+    log(f"bench: issue rates {rates}")
 
     # the outputs of every mode, for the CPU comparison
     gpu_out = []
@@ -578,7 +916,7 @@ def main():
                 if name == "div":
                     return lib.cft_div(dev.handle, 3, rnd, bufs[0].data_ptr, bufs[1].data_ptr,
                                        bufs[3].data_ptr, n, None, None)
-                return lib.cft_sqrt(dev.handle, 3, rnd, bufs[0].data_ptr, bufs[3].data_ptr,
+                return lib.cft_sqrt(dev.handle, 3, rnd, bufs[4].data_ptr, bufs[3].data_ptr,
                                     n, None, None)
             # one untimed call first: a kernel's first launch also loads its
             # code object, which would land between the timer's events
@@ -595,6 +933,22 @@ def main():
             ops[name] = {"Gelem_s": round(5 * n / dt / 1e9, 4),
                          "kernel_ms": round(ks * 1e3, 4),
                          "kernel_GBs": round(BYTES[name] * n / ks / 1e9, 1)}
+            if name == "sqrt":
+                ops[name]["operand"] = "|a| (the sign bit of a cleared)"
+            if not args.skip_cpu and ref.ok:
+                sample = name in ("div", "sqrt")
+                t1 = time.perf_counter()
+                tot = bad = 0
+                for rnd in range(5):
+                    k, b_ = check_op(lib, dev, bufs, n, name, rnd, ref, threads, sample)
+                    tot += k
+                    bad += b_
+                ops[name]["vs_reference_cpu"] = {
+                    "elements_checked": tot, "differing": bad, "bit_exact": bad == 0,
+                    "checked": ("every 16th element of each of the 5 modes, with the GPU's "
+                                "own call over the gathered elements for the flags")
+                    if sample else "every element of all 5 modes, flags included",
+                    "seconds": round(time.perf_counter() - t1, 1)}
             log(f"bench: {name} {ops[name]}")
         # reductions over the same resident binary256 operands: the tree over
         # the whole array, and the same over segments (one result a segment)
@@ -665,33 +1019,32 @@ def main():
                                "kernel_ms": round(ks * 1e3, 4),
                                "kernel_Gelem_s": round(n1 / ks / 1e9, 4),
                                "kernel_GBs": round(64 * n1 / ks / 1e9, 1), "flags": f1.value}
-        if not args.skip_cpu:
-            r = ref_elem(b1[0].host, b1[1].host, b1[2].host, n1, 2, 0, 16)
-            if r is not None:
-                ops["fma_fp128_c1"]["bit_exact_vs_cpu"] = bool(
-                    np.array_equal(r[0], b1[3].host) and r[1] == f1.value)
-                ops["fma_fp128_c1"]["cpu_Gelem_s"] = round(n1 / r[2] / 1e9, 5)
+        if not args.skip_cpu and ref.ok:
+            r = ref.elem("fma", 2, 0, b1[0].host, b1[1].host, b1[2].host, n1, threads)
+            ops["fma_fp128_c1"]["bit_exact_vs_cpu"] = bool(
+                np.array_equal(r[0], b1[3].host) and r[1] == f1.value)
+            ops["fma_fp128_c1"]["cpu_Gelem_s"] = round(n1 / r[2] / 1e9, 5)
+            ops["fma_fp128_c1"]["cpu_threads"] = threads
         log(f"bench: fp128 FMA {ops['fma_fp128_c1']}")
         for bf in b1:
             bf.free()
 
     rk4 = None
     if not args.skip_rk4:
-        rk4 = rk4_section(dev, lib, args, args.cpu_threads or min(16, os.cpu_count() or 1))
+        rk4 = rk4_section(dev, lib, args, threads, ref)
 
     serve = None
     if not args.skip_serve:
-        serve = serve_section(args)
+        serve = serve_section(args, threads, ref)
 
     cpu = None
     bit_exact = None
     if not args.skip_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         a = bufs[0].host.copy()
         b = bufs[1].host.copy()
         c = bufs[2].host.copy()
         log(f"bench: CPU baseline on {threads} threads")
-        cpu, mism = cpu_baseline(a, b, c, gpu_out, n, threads)
+        cpu, mism = cpu_baseline(a, b, c, gpu_out, n, threads, ref, info)
         if cpu is not None:
             bit_exact = mism == 0
             log(f"bench: CPU {cpu['value']} Gelem/s ({cpu['kind']}), mismatches {mism}")
@@ -700,6 +1053,37 @@ def main():
     if stream:
         log(f"bench: torch.addcmul on the same bytes streams {stream:.0f} GB/s; the FMA kernel "
             f"moves {achieved:.0f} GB/s ({achieved / stream:.2f} of it)")
+    kernel_gelem = n / kavg_s / 1e9
+    valu = None
+    if rates:
+        mad_rel = rates["relative_to_v_add_u32"]["v_mad_u64_u32"]
+        slots = FMA_MACS / mad_rel
+        valu = {
+            "valu_issue": round(FMA_VALU_PER_ELEM * (n / 64) / kavg_s / VALU_PEAK_WAVE_INSNS, 4),
+            "valu_instr_per_elem": FMA_VALU_PER_ELEM,
+            "alg_valu_frac": round(kernel_gelem * 1e9 * slots / VALU_PEAK_LANE_OPS, 4),
+            "alg_slots_per_elem": round(slots, 1),
+            "peak_wave_insns_s": VALU_PEAK_WAVE_INSNS,
+            "measured_v_add_u32_wave_insns_s": round(
+                rates["wave_insns_per_s"]["v_add_u32"] * 1e9, 1),
+            "mad_u64_u32_rate_vs_add": mad_rel,
+            "issue_rates": rates,
+            "valu_source": "valu_instr_per_elem: rocprofv3 SQ_INSTS_VALU / SQ_WAVES of this "
+                           "kernel on this workload (profiles/elementwise_ops_valu.md); "
+                           "valu_issue = that x elements/64 / kernel time / peak; "
+                           "alg_valu_frac = in-kernel elem/s x 64 MACs at their measured issue "
+                           "cost (slots) / 39.3e12 lane-ops/s"}
+        log(f"bench: VALU {valu['valu_issue']} issue, alg {valu['alg_valu_frac']}")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(FMA_TRAFFIC_PER_ELEM * n),
+            "kernel_ms": round(kavg_s * 1e3, 4),
+            "kernel_Gelem_s": round(kernel_gelem, 4),
+            "bytes_per_elem": FMA_BYTES,
+            "stream_GBs": round(stream, 1) if stream else None,
+            "frac_of_stream": round(achieved / stream, 4) if stream else None}
+    if valu:
+        roof.update(valu)
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -716,14 +1100,7 @@ def main():
         "config": {"workload": "binary256 FMA over the elementwise configuration's "
                                f"operands, N={n}, seed {args.seed}, 5 rounding modes a step",
                    "elements_per_step": 5 * n, "format": "binary256", "resident": True},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(FMA_TRAFFIC_PER_ELEM * n),
-                     "kernel_ms": round(kavg_s * 1e3, 4),
-                     "kernel_Gelem_s": round(n / kavg_s / 1e9, 4),
-                     "bytes_per_elem": FMA_BYTES,
-                     "stream_GBs": round(stream, 1) if stream else None,
-                     "frac_of_stream": round(achieved / stream, 4) if stream else None},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "bit_exact_vs_cpu": bit_exact,
         "ops": ops,

@@ -352,3 +352,55 @@ class Buffer:
             self.free()
         except Exception:
             pass
+
+
+class DeviceArray:
+    """Raw device memory of the handle's GPU (hipMalloc of the HIP runtime
+    libcft itself is linked against), for job windows that need no host
+    mirror: the serving loop takes such windows as well as resident ones.
+    ``ptr`` is the device address; ``download`` copies bytes back."""
+
+    _rt = None
+
+    @classmethod
+    def runtime(cls):
+        if cls._rt is None:
+            lib()                                 # libcft loads the runtime first
+            rt = ctypes.CDLL("libamdhip64.so.7")
+            rt.hipMalloc.argtypes = [ctypes.POINTER(c_void_p), ctypes.c_size_t]
+            rt.hipFree.argtypes = [c_void_p]
+            rt.hipMemcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, ctypes.c_int]
+            rt.hipMemset.argtypes = [c_void_p, ctypes.c_int, ctypes.c_size_t]
+            cls._rt = rt
+        return cls._rt
+
+    def __init__(self, nbytes, fill=None):
+        rt = self.runtime()
+        p = c_void_p()
+        st = rt.hipMalloc(byref(p), max(1, nbytes))
+        if st != 0:
+            raise MemoryError(f"hipMalloc of {nbytes} bytes failed ({st})")
+        self.ptr, self.nbytes = p.value, nbytes
+        if fill is not None and rt.hipMemset(self.ptr, fill, nbytes) != 0:
+            raise RuntimeError("hipMemset failed")
+
+    def download(self, offset=0, nbytes=None, out=None):
+        nbytes = self.nbytes - offset if nbytes is None else nbytes
+        if offset < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("window outside the allocation")
+        if out is None:
+            out = np.empty(nbytes, np.uint8)
+        if self.runtime().hipMemcpy(out.ctypes.data, self.ptr + offset, nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy device to host failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.runtime().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -78,21 +78,27 @@ def lorenz96(n, seed=0, fmt=3, steps_lo=1000, steps_hi=20000, ks=None, forcing=8
 
 
 def henonheiles(n, seed=0, fmt=3, steps_lo=10000, steps_hi=100000, h="0.01", seg=100,
-                energy=0.125):
-    """n orbits at energy E: x, y, py uniform, px >= 0 from the energy
-    (computed in binary64 from the drawn values; the state is what the
-    program integrates, whatever px's last bit)."""
+                energy="1/8"):
+    """n orbits at energy E: x, y, py drawn uniform in [-0.2, 0.2] (binary64
+    values, exact in every wider format), px = RN(sqrt(2E - py^2 - x^2 - y^2
+    - 2 x^2 y + 2 y^3 / 3)) in the format itself, the argument evaluated
+    exactly as a rational and its square root correctly rounded, so the
+    orbit's energy is E to the format's own precision."""
+    from fractions import Fraction
     rng = np.random.default_rng(seed)
-    out = np.empty((n, 4))
-    i = 0
-    while i < n:
+    esz = ESZ[fmt]
+    E = Fraction(energy)
+    rows = []
+    while len(rows) < n:
         x, y, py = rng.uniform(-0.2, 0.2, size=3)
-        rest = 2 * energy - py * py - x * x - y * y - 2 * x * x * y + 2 * y ** 3 / 3
+        fx, fy, fpy = Fraction(float(x)), Fraction(float(y)), Fraction(float(py))
+        rest = 2 * E - fpy * fpy - fx * fx - fy * fy - 2 * fx * fx * fy + 2 * fy ** 3 / 3
         if rest <= 0:
             continue
-        out[i] = (x, y, np.sqrt(rest), py)
-        i += 1
+        enc = widen(np.array([x, y, py]), fmt).reshape(3, esz)
+        px = np.frombuffer(P.round_sqrt(fmt, rest).to_bytes(esz, "little"), dtype=np.uint8)
+        rows.append(np.concatenate([enc[0], enc[1], px, enc[2]]))
     steps = rng.integers(steps_lo, steps_hi + 1, size=n).astype(np.uint32)
     img = P.with_trips(P.henonheiles(fmt), seg)
     bank = np.frombuffer(P.bank_bytes("henonheiles-lf", fmt, h), dtype=np.uint8).copy()
-    return img, bank, widen(out.reshape(-1), fmt), steps
+    return img, bank, np.concatenate(rows).copy(), steps

@@ -195,6 +195,31 @@ def to_rational(fmt, bits):
     return -v if sign else v
 
 
+def round_sqrt(fmt, q):
+    """RN-even of sqrt(q) for a positive rational q (Fraction) in format fmt,
+    as its encoding (normal results only).  An integer square root gives
+    p + 2 bits of the root exactly, m, at scale 2^k; when the root is not
+    exactly m * 2^k it lies strictly between m and m + 1, where every
+    rounding decision is the same as for m + 1/2, so the rounding of that
+    rational is the rounding of the root."""
+    from math import isqrt
+    _width, p, _ew = FORMATS[fmt]
+    assert q > 0
+    num, den = q.numerator, q.denominator
+    e = (num.bit_length() - den.bit_length()) // 2 - 1
+    while Fraction(2) ** (2 * (e + 1)) <= q:
+        e += 1
+    while Fraction(2) ** (2 * e) > q:
+        e -= 1
+    # 2^e <= sqrt(q) < 2^(e+1); m = floor(sqrt(q) * 2^(p+1-e)) has p+2 bits
+    s = p + 1 - e
+    scaled = q * Fraction(2) ** (2 * s)
+    m = isqrt(scaled.numerator // scaled.denominator)
+    exact = Fraction(m * m) == scaled
+    v = Fraction(m) if exact else Fraction(2 * m + 1, 2)
+    return round_rational(fmt, v / Fraction(2) ** s)
+
+
 def dec(fmt, text):
     return round_rational(fmt, Fraction(text))
 

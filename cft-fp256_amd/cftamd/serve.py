@@ -28,7 +28,7 @@ import numpy as np
 from . import _lib, odes, workload
 from . import programs as P
 from ._lib import check, lib
-from .core import Program, run_multi
+from .core import DeviceArray, Program, run_multi
 
 KIND_NAMES = {0: "fma", 1: "div", 2: "sqrt", 3: "lorenz63", 4: "lorenz96", 5: "henonheiles"}
 
@@ -125,8 +125,10 @@ def requests(seed, rate, duration, pool_n=10_000_000, mix=(0.8, 0.1, 0.05, 0.05)
 class Pools:
     """Device-resident operand pools, output windows and ODE programs."""
 
-    def __init__(self, dev, reqs, seed=0, pool_n=10_000_000, verify_every=25):
+    def __init__(self, dev, reqs, seed=0, pool_n=10_000_000, verify_every=25,
+                 device_out=False):
         self.dev, self.pool_n = dev, pool_n
+        self.device_out = device_out
         self.ops = {}
         for fmt, cfg in ((2, 1), (3, 2)):
             esz = 16 if fmt == 2 else 32
@@ -141,16 +143,25 @@ class Pools:
                 bf.to_device()
             self.ops[fmt] = bufs
         # outputs: every `verify_every`-th elementwise job writes a window of
-        # its own, kept for the bit check; the others share a sink
+        # its own, kept for the bit check; the others share a sink.  With
+        # device_out every job has a window of its own in raw device memory
+        # (no pinned host mirror of the whole stream's output)
+        if device_out:
+            verify_every = 1
         need = {2: 0, 3: 0}
         for i, r in enumerate(reqs):
             if r["kind"] <= 2 and i % verify_every == 0:
                 need[r["fmt"]] += r["n"]
-        self.out = {f: dev.alloc(max(1, need[f]) * (16 if f == 2 else 32)) for f in (2, 3)}
-        self.sink = {f: dev.alloc(pool_n * (16 if f == 2 else 32)) for f in (2, 3)}
-        for f in (2, 3):
-            self.out[f].to_device()
-            self.sink[f].to_device()
+        esz_of = {2: 16, 3: 32}
+        if device_out:
+            self.out = {f: DeviceArray(max(1, need[f]) * esz_of[f], fill=0xA5) for f in (2, 3)}
+            self.sink = {}
+        else:
+            self.out = {f: dev.alloc(max(1, need[f]) * esz_of[f]) for f in (2, 3)}
+            self.sink = {f: dev.alloc(pool_n * esz_of[f]) for f in (2, 3)}
+            for f in (2, 3):
+                self.out[f].to_device()
+                self.sink[f].to_device()
         pos = {2: 0, 3: 0}
         for i, r in enumerate(reqs):
             if r["kind"] <= 2:
@@ -205,6 +216,19 @@ class Pools:
     def bank_of(self, r):
         return {3: self.banks["l63"], 4: self.banks["l96"], 5: self.banks["hh"]}[r["kind"]]
 
+    def out_ptr(self, fmt):
+        o = self.out[fmt]
+        return o.ptr if self.device_out else o.data_ptr
+
+    def served(self, r):
+        """A kept elementwise job's output bytes (after the stream ran; for
+        resident outputs after their from_device)."""
+        esz = 16 if r["fmt"] == 2 else 32
+        lo, nb = r["out"] * esz, r["n"] * esz
+        if self.device_out:
+            return self.out[r["fmt"]].download(lo, nb)
+        return self.out[r["fmt"]].host[lo:lo + nb]
+
     def free(self):
         for bufs in self.ops.values():
             for b in bufs:
@@ -219,7 +243,7 @@ def submit(srv, pools, r):
     if r["kind"] <= 2:
         fmt, esz = r["fmt"], (16 if r["fmt"] == 2 else 32)
         a, b, c, sa = (x.data_ptr + r["off"] * esz for x in pools.ops[fmt])
-        d = (pools.out[fmt].data_ptr + r["out"] * esz if r["keep"] else
+        d = (pools.out_ptr(fmt) + r["out"] * esz if r["keep"] else
              pools.sink[fmt].data_ptr)
         if r["kind"] == 0:
             return srv.submit_elem(_lib.JOB_ELEM, 0, fmt, 0, a, b, c, d, r["n"])
@@ -276,19 +300,20 @@ def summarize(reqs, results, wall):
     return out
 
 
-def verify(dev, pools, reqs, results, alone_every=1):
+def verify(dev, pools, reqs, results, alone_every=1, hook=None):
     """Every job against the same job run on its own: flags for every
-    elementwise job (cft_run / cft_div / cft_sqrt), bits for every
-    `verify_every`-th; final states, flags and STATUS for every program job.
-    Every `alone_every`-th program job is rerun by itself
-    (cftx_program_run_batch); the others in one cftx_program_run_multi
-    launch, a part each (a lane's results depend on its own state and trips
-    only)."""
+    elementwise job (cft_run / cft_div / cft_sqrt), bits for every kept one
+    (every `verify_every`-th, or all of them with device_out); final states,
+    flags and STATUS for every program job.  Every `alone_every`-th program
+    job is rerun by itself (cftx_program_run_batch); the others in one
+    cftx_program_run_multi launch, a part each (a lane's results depend on
+    its own state and trips only).  hook(r, result, served_bytes) sees every
+    kept elementwise job's output."""
     L = lib()
-    for b in pools.out.values():
-        b.from_device()
+    if not pools.device_out:
+        for b in pools.out.values():
+            b.from_device()
     pools.states.from_device()
-    served_out = {f: pools.out[f].host.copy() for f in (2, 3)}
     served_states = pools.states.host.copy()
     scratch = {f: dev.alloc(pools.pool_n * (16 if f == 2 else 32)) for f in (2, 3)}
     bad = checked = bits = alone = 0
@@ -309,8 +334,11 @@ def verify(dev, pools, reqs, results, alone_every=1):
             else:
                 st = L.cft_sqrt(dev.handle, fmt, 0, sa, d, r["n"], ctypes.byref(fl), None)
             ok = st == 0 and fl.value == res.flags
-            if ok and r["keep"]:
-                ok = np.array_equal(host, served_out[fmt][r["out"] * esz:(r["out"] + r["n"]) * esz])
+            if r["keep"]:
+                served = pools.served(r)
+                if hook is not None:
+                    hook(r, res, served)
+                ok = ok and np.array_equal(host, served)
                 bits += 1
             bad += 0 if ok else 1
             continue

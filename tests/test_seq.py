@@ -110,6 +110,44 @@ def test_round_rational_is_nearest_even(fmt):
     assert P.round_rational(fmt, 1 + 3 * ulp1 / 2) == one + 2         # tie to even (up)
 
 
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_round_sqrt_brackets_the_root(fmt):
+    """RN(sqrt(q)) lies between the midpoints to its neighbours, squared (the
+    lower neighbour half as far at a power of two); perfect squares are
+    exact; binary64 agrees with the platform's correctly rounded sqrt."""
+    import math
+    width, p, ew = P.FORMATS[fmt]
+    bias = (1 << (ew - 1)) - 1
+    rng = np.random.default_rng(40 + fmt)
+    for _ in range(300):
+        q = Fraction(int(rng.integers(1, 2**62)), int(rng.integers(1, 2**62))) * \
+            Fraction(2) ** int(rng.integers(-30, 31))
+        v = P.round_sqrt(fmt, q)
+        r = P.to_rational(fmt, v)
+        e = (v >> (p - 1)) & ((1 << ew) - 1)
+        ulp = Fraction(2) ** (e - bias - (p - 1))
+        below = ulp / 2 if (v & ((1 << (p - 1)) - 1)) == 0 else ulp
+        assert (r - below / 2) ** 2 <= q <= (r + ulp / 2) ** 2, q
+        if fmt == 1:
+            assert v == struct.unpack("<Q", struct.pack("<d", math.sqrt(float(q))))[0] or \
+                float(q) != q
+    for k in (k for k in (1, 3, 12345, 2**40 + 1) if k.bit_length() <= p):
+        assert P.to_rational(fmt, P.round_sqrt(fmt, Fraction(k * k, 4 ** 7))) == \
+            Fraction(k, 2 ** 7)
+
+
+def test_henon_heiles_orbits_at_the_energy():
+    """Orbit states carry E = 1/8 to binary256 precision: px is the
+    correctly rounded root, so H(x, y, px, py) - 1/8 is within a few ulp of
+    px^2 / 2 (about 2^-237 relative)."""
+    from cftamd import odes
+    _img, _bank, st, _steps = odes.henonheiles(8, seed=3, fmt=3)
+    for row in st.reshape(8, 4, 32):
+        x, y, px, py = (P.to_rational(3, int.from_bytes(v.tobytes(), "little")) for v in row)
+        H = (px * px + py * py) / 2 + (x * x + y * y) / 2 + x * x * y - y ** 3 / 3
+        assert abs(H - Fraction(1, 8)) <= Fraction(1, 2 ** 230)
+
+
 # ---- the oracle's executor ---------------------------------------------------------
 
 def cert_run(oracle_or_none, name, run, runner):

@@ -64,13 +64,14 @@ def mixed_requests(seed, pool_n, per=2, duration=0.2):
 def test_every_served_job_against_the_oracle(deep, oracle):
     """Each served job's output bits and flags against the oracle on the
     same operands: elementwise FMA / DIV / SQRT in binary128 and binary256
-    (each job its own output window), and the final states, flags and STATUS
+    (each job its own output window, raw device memory), and the final
+    states, flags and STATUS
     of the Lorenz-63, Lorenz-96 and Henon-Heiles jobs against the oracle's
     chain of segments.  Programs run shortened (steps / 200) so the oracle
     finishes in seconds."""
     pool_n = 20_000
     reqs = mixed_requests(11, pool_n)
-    pools = S.Pools(deep, reqs, seed=5, pool_n=pool_n, verify_every=1)
+    pools = S.Pools(deep, reqs, seed=5, pool_n=pool_n, device_out=True)
     for r in reqs:
         if r["kind"] >= 3:
             r["steps"] = np.maximum(1, np.asarray(r["steps"]) // 200).astype(np.uint32)
@@ -78,8 +79,6 @@ def test_every_served_job_against_the_oracle(deep, oracle):
     res, _wall = S.serve(srv, pools, reqs)
     srv.close()
     assert all(r.status == 0 for r in res)
-    for b in pools.out.values():
-        b.from_device()
     pools.states.from_device()
     kinds = set()
     for r, got in zip(reqs, res):
@@ -93,7 +92,7 @@ def test_every_served_job_against_the_oracle(deep, oracle):
                 want, wfl = oracle_elem(oracle, fmt, 256, 0, a, b, None)
             else:
                 want, wfl = oracle_elem(oracle, fmt, 257, 0, sa, None, None)
-            out = pools.out[fmt].host[r["out"] * esz:(r["out"] + n) * esz]
+            out = pools.served(r)
             assert np.array_equal(out, want), (r, np.flatnonzero(out != want)[:4])
             assert got.flags == wfl, (r, got.flags, wfl)
             kinds.add((r["kind"], fmt))
