@@ -69,14 +69,19 @@ FMA_BYTES = 128          # 3 x 32 B in, 32 B out per element
 # plain 256 MiB copy (profiles/fma_fp256_elementwise.md)
 FMA_TRAFFIC_PER_ELEM = (393543.0 + 131129.0) * 1024 / 4194304
 BYTES = {"add": 96, "mul": 96, "fma": 128, "div": 96, "sqrt": 64}
-# VALU: 256 CUs x 4 SIMDs x 2.4 GHz, a wave64 instruction every 4 cycles
-VALU_PEAK_WAVE_INSNS = 256 * 4 * 2.4e9 / 4
-VALU_PEAK_LANE_OPS = VALU_PEAK_WAVE_INSNS * 64
+# VALU: 256 CUs x 4 SIMDs x 2.4 GHz.  gfx950 issues a plain 32-bit wave64
+# operation every 2 cycles (32 lanes a cycle: the 157.3 TFLOP/s vector fp32
+# rate) and a carry, multiply or v_mad_u64_u32 every 4 (the probes measure
+# 2.9 and 4.3-4.7 cycles with loop control, profiles/valu_issue_rates.md)
+VALU_PEAK_WAVE_INSNS = 256 * 4 * 2.4e9 / 2
+VALU_4CYCLE_WAVE_INSNS = 256 * 4 * 2.4e9 / 4
 # VALU instructions a lane issues for one binary256 FMA element of this
-# workload (rocprofv3 SQ_INSTS_VALU / SQ_WAVES at 2^22 elements,
-# profiles/elementwise_ops_valu.md); the exact product's 32x32->64
-# multiply-accumulates an element (8 x 8 limbs)
+# workload (rocprofv3 SQ_INSTS_VALU / SQ_WAVES at 2^22 elements, and the
+# cycles a SIMD spends a VALU instruction there, GRBM_GUI_ACTIVE / 8 XCDs
+# over SQ_INSTS_VALU / 1,024 SIMDs: profiles/valu_issue_rates.md); the exact
+# product's 32x32->64 multiply-accumulates an element (8 x 8 limbs)
 FMA_VALU_PER_ELEM = 854.6
+FMA_CYCLES_PER_VALU = 4.16
 FMA_MACS = 64
 
 
@@ -1056,24 +1061,30 @@ def main():
     kernel_gelem = n / kavg_s / 1e9
     valu = None
     if rates:
-        mad_rel = rates["relative_to_v_add_u32"]["v_mad_u64_u32"]
-        slots = FMA_MACS / mad_rel
+        wave_insns = FMA_VALU_PER_ELEM * (n / 64) / kavg_s
+        mad_rate = rates["wave_insns_per_s"]["v_mad_u64_u32"] * 1e9
         valu = {
-            "valu_issue": round(FMA_VALU_PER_ELEM * (n / 64) / kavg_s / VALU_PEAK_WAVE_INSNS, 4),
+            "valu_issue": round(wave_insns / VALU_PEAK_WAVE_INSNS, 4),
+            "valu_issue_vs_4cycle_rate": round(wave_insns / VALU_4CYCLE_WAVE_INSNS, 4),
+            "valu_issue_vs_measured_add": round(
+                wave_insns / (rates["wave_insns_per_s"]["v_add_u32"] * 1e9), 4),
+            "valu_wave_insns_s": round(wave_insns, 1),
             "valu_instr_per_elem": FMA_VALU_PER_ELEM,
-            "alg_valu_frac": round(kernel_gelem * 1e9 * slots / VALU_PEAK_LANE_OPS, 4),
-            "alg_slots_per_elem": round(slots, 1),
+            "valu_cycles_per_instr_per_simd": FMA_CYCLES_PER_VALU,
+            "alg_valu_frac": round(kernel_gelem * 1e9 * FMA_MACS / 64 / mad_rate, 4),
             "peak_wave_insns_s": VALU_PEAK_WAVE_INSNS,
-            "measured_v_add_u32_wave_insns_s": round(
-                rates["wave_insns_per_s"]["v_add_u32"] * 1e9, 1),
-            "mad_u64_u32_rate_vs_add": mad_rel,
+            "mad_u64_u32_wave_insns_s": round(mad_rate, 1),
             "issue_rates": rates,
-            "valu_source": "valu_instr_per_elem: rocprofv3 SQ_INSTS_VALU / SQ_WAVES of this "
-                           "kernel on this workload (profiles/elementwise_ops_valu.md); "
-                           "valu_issue = that x elements/64 / kernel time / peak; "
-                           "alg_valu_frac = in-kernel elem/s x 64 MACs at their measured issue "
-                           "cost (slots) / 39.3e12 lane-ops/s"}
-        log(f"bench: VALU {valu['valu_issue']} issue, alg {valu['alg_valu_frac']}")
+            "valu_source": "valu_instr_per_elem and cycles a VALU instruction: rocprofv3 "
+                           "SQ_INSTS_VALU, SQ_WAVES, GRBM_GUI_ACTIVE of this kernel on this "
+                           "workload (profiles/valu_issue_rates.md); valu_issue = that x "
+                           "elements/64 / kernel time / (256 CUs x 4 SIMDs x 2.4 GHz / 2 "
+                           "cycles); alg_valu_frac = the exact product's 64 MACs an element "
+                           "at the measured v_mad_u64_u32 issue rate: (elem/s x 64 / 64 "
+                           "lanes) / that rate"}
+        log(f"bench: VALU {valu['valu_issue']} of the 2-cycle peak "
+            f"({valu['valu_issue_vs_4cycle_rate']} of the 4-cycle rate), "
+            f"alg {valu['alg_valu_frac']}")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": round(FMA_TRAFFIC_PER_ELEM * n),

@@ -8,6 +8,15 @@
                                         a reduction (SUM 24, DOT 25, SUMSQ 28,
                                         SUMABS 29, MAXALL 31) over the FMA
                                         operands (binary128 or binary256), 5 modes
+    python -m cftamd.run_workload issue    the issue-rate probes (lib/libcftprobe.so)
+    python -m cftamd.run_workload kernels  the other kernels once each: k_misc
+                                        (binary256 MIN over the FMA operands,
+                                        --n elements), k_seq (one Lorenz-63
+                                        segment of 100 steps over 65,536 lanes),
+                                        k_seq_batch (the ensemble's first 16,384
+                                        trajectories at steps // 100, one lane an
+                                        instance) and k_seq_op (the plain form
+                                        on 4,096 of them at steps // 1,000)
 
 Each builds its inputs as bench.py does (same generators and seeds), runs
 the workload once on GPU 0 with every input resident in device memory, and
@@ -19,6 +28,8 @@ import argparse
 import ctypes
 import sys
 import time
+
+import numpy as np
 
 from . import Device, Program, _lib, odes, run_multi, workload
 
@@ -135,9 +146,65 @@ def serve(args):
         pools.free()
 
 
+def issue(args):
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
+                        "libcftprobe.so")
+    P = ctypes.CDLL(path)
+    P.cftp_issue_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]
+    for kind, name in enumerate(("v_add_u32", "v_mad_u64_u32", "v_mul_lo_u32",
+                                 "v_mul_hi_u32", "v_add_co_u32+v_addc_co_u32")):
+        ms, wi = ctypes.c_double(), ctypes.c_double()
+        if P.cftp_issue_rate(kind, 2048, 16384, 5, ctypes.byref(ms), ctypes.byref(wi)) != 0:
+            raise RuntimeError("the issue-rate probe failed")
+        print(f"issue {name}: {wi.value / (ms.value * 1e-3) / 1e9:.1f} G wave instructions/s "
+              f"({ms.value:.3f} ms a launch)")
+
+
+def kernels(args):
+    lib = _lib.lib()
+    n = args.n
+    with Device("hip:0") as dev:
+        bufs = [dev.alloc(n * 32) for _ in range(3)]
+        workload.elementwise(2, n, seed=args.seed, threads=16,
+                             out=(bufs[0].host, bufs[1].host, bufs[2].host))
+        for b in bufs[:2]:
+            b.to_device()
+        fl = ctypes.c_uint32()
+        for _ in range(3):
+            st = lib.cft_run(dev.handle, 7, 3, 0, bufs[0].data_ptr, bufs[1].data_ptr, None,
+                             bufs[2].data_ptr, n, ctypes.byref(fl), None)
+            assert st == 0, lib.cft_last_error()
+        print(f"k_misc: binary256 MIN over {n} elements x 3, flags {fl.value}")
+        for b in bufs:
+            b.free()
+        img, bank, state, steps = odes.lorenz63(65536, seed=args.seed, fmt=3)
+        prog = Program(dev, img)
+        lanes = steps.size
+        za = np.zeros(lanes * 32, np.uint8)
+        sout = np.empty_like(state)
+        f, s = prog.run(lanes, za, bank=bank, scratch_in=state, scratch_out=sout)
+        print(f"k_seq: one 100-step segment over {lanes} lanes, flags {f}")
+        m = 16384
+        st2 = state[: m * 96].copy()
+        f, s = prog.run_batch(st2, np.maximum(1, steps[:m] // 100), bank=bank,
+                              lanes_per_instance=1)
+        print(f"k_seq_batch: {m} trajectories at steps // 100, one lane an instance, flags {f}")
+        m = 4096
+        st3 = state[: m * 96].copy()
+        f, s = prog.run_batch(st3, np.maximum(1, steps[:m] // 1000), bank=bank,
+                              mode=_lib.BATCH_PER_OP)
+        print(f"k_seq_op: {m} trajectories at steps // 1000, "
+              f"{int(lib.cftx_batch_launches(dev.handle))} launches, flags {f}")
+        prog.free()
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m cftamd.run_workload")
-    ap.add_argument("workload", choices=["fma", "l63", "l96", "serve", "reduce"])
+    ap.add_argument("workload", choices=["fma", "l63", "l96", "serve", "reduce", "issue",
+                                         "kernels"])
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--fmt", type=int, default=3, choices=[1, 2, 3])
     ap.add_argument("--seg", type=int, default=0)
@@ -145,8 +212,8 @@ def main(argv=None):
     ap.add_argument("--seconds", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
-    {"fma": fma, "l63": l63, "l96": l96, "serve": serve,
-     "reduce": reduce}[args.workload](args)
+    {"fma": fma, "l63": l63, "l96": l96, "serve": serve, "reduce": reduce, "issue": issue,
+     "kernels": kernels}[args.workload](args)
     return 0
 
 

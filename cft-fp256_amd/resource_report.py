@@ -64,6 +64,15 @@ def main():
             if m and cur is not None:
                 cur[m.group(1).strip()] = m.group(2)
     names = demangle([r["name"] for r in rows])
+    print("# Per-kernel resources (gfx950, from the compiler)\n")
+    print("Every kernel of libcft.so and lib/libcftprobe.so, compiled with the library's "
+          "device flags and\n`-Rpass-analysis=kernel-resource-usage` (hipcc, ROCm 7.2). "
+          "Command, from the repository root:\n")
+    print("    python3 cft-fp256_amd/resource_report.py > profiles/kernel_resources.md\n")
+    print("waves/SIMD is the compiler's bound from registers and static LDS at the "
+          "launch bounds; the\nsequencer kernels also take dynamic LDS sized at launch "
+          "(cftk_seq_lds, cftk_seq_wide_lds,\ncftk_seq_multi_lds), which bounds them "
+          "lower per program.  No kernel spills (scratch 0).\n")
     print("| source | kernel | " + " | ".join(h for _k, h in FIELDS) + " |")
     print("|---|---|" + "---|" * len(FIELDS))
     for r, n in zip(rows, names):
