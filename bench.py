@@ -394,17 +394,25 @@ def rk4_section(dev, lib, args, threads, ref):
     bbuf = dev.alloc(bank.nbytes)
     bbuf.host[:] = bank
     bbuf.to_device()
+    # the autotuner on a 1/100 sample of the steps: lanes an instance (1 to
+    # 16, or the library's model) and workgroup size, by measurement
+    from cftamd import tune_batch
+    prog.prepare(0x7F)
+    s_tune = np.maximum(1, steps // 100).astype(np.uint32)
+    knobs, trials = tune_batch(prog, state, s_tune, bank=bank)
+    log(f"bench: rk4 tuned {knobs} ({len(trials)} trials, {trials[0][0]:.4f} .. "
+        f"{trials[-1][0]:.4f} s)")
     # warm the graph on this exact launch shape: a pass over a copy of the
     # ensemble with one step a lane, then the timed pass over the real steps
     sbuf.host[:] = state
     sbuf.to_device()
-    prog.run_batch(sbuf, np.ones_like(steps), bank=bbuf, graph=True)
+    prog.run_batch(sbuf, np.ones_like(steps), bank=bbuf, graph=True, **knobs)
     sbuf.from_device()
     sbuf.host[:] = state
     sbuf.to_device()
     lib.cftx_kernel_timer(dev.handle, 1)
     t0 = time.perf_counter()
-    fl, st = prog.run_batch(sbuf, steps, bank=bbuf, graph=True)
+    fl, st = prog.run_batch(sbuf, steps, bank=bbuf, graph=True, **knobs)
     secs = time.perf_counter() - t0
     kms, kl = ctypes.c_double(), ctypes.c_uint64()
     lib.cftx_kernel_timer_read(dev.handle, ctypes.byref(kms), ctypes.byref(kl))
@@ -417,7 +425,9 @@ def rk4_section(dev, lib, args, threads, ref):
         "persistent_s": round(secs, 4), "kernel_s": round(kms.value / 1e3, 4),
         "steps_per_s": round(total / secs, 1), "alu_ops_per_s": round(53 * total / secs, 1),
         "flags": fl, "status": st, "launches": int(lib.cftx_batch_launches(dev.handle)),
-        "lanes_per_instance": int(lib.cftx_batch_width(dev.handle))}
+        "lanes_per_instance": int(lib.cftx_batch_width(dev.handle)),
+        "tuned_knobs": knobs, "tuning_sample": "steps // 100 (at least 1)",
+        "tuning_trials_s": [[t, k] for t, k in trials]}
     log(f"bench: rk4 persistent {secs:.3f} s -> {total / secs / 1e6:.1f} M steps/s")
 
     # the plain per-op form on a 1/100 sample, beside the persistent form
@@ -426,7 +436,7 @@ def rk4_section(dev, lib, args, threads, ref):
     ta.host[:] = state
     ta.to_device()
     t0 = time.perf_counter()
-    prog.run_batch(ta, s2, bank=bbuf, graph=True)
+    prog.run_batch(ta, s2, bank=bbuf, graph=True, **knobs)
     tp = time.perf_counter() - t0
     tb = dev.alloc(state.nbytes)
     tb.host[:] = state
@@ -481,7 +491,7 @@ def rk4_section(dev, lib, args, threads, ref):
 
     # Lorenz-96: 8,192 instances, ring 8..256, every ring size in one launch;
     # the planner's knobs tuned on a 1/100 sample of the steps first
-    from cftamd import last_plan, run_multi, tune_multi
+    from cftamd import apply_knobs, last_plan, run_multi, tune_multi
     with Device("hip:0", scratch_depth=2048) as d2:
         for fmt, name in ((1, "lorenz96_fp64"), (3, "lorenz96_fp256")):
             groups = odes.lorenz96(8192, seed=args.seed, fmt=fmt)
@@ -493,10 +503,10 @@ def rk4_section(dev, lib, args, threads, ref):
             samp = [np.maximum(1, g[5] // 100).astype(np.uint32) for g in groups]
             knobs, trials = tune_multi(d2, [dict(prog=p, bank=g[2], state=g[4], steps=s2)
                                             for p, g, s2 in zip(progs, groups, samp)])
-            parts = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=g[5])
-                     for p, g in zip(progs, groups)]
+            parts, kw = apply_knobs([dict(prog=p, bank=g[2], state=g[4].copy(), steps=g[5])
+                                     for p, g in zip(progs, groups)], knobs)
             t0 = time.perf_counter()
-            fl, _s, widths = run_multi(d2, parts, **knobs)
+            fl, _s, widths = run_multi(d2, parts, **kw)
             secs = time.perf_counter() - t0
             plan = last_plan(d2)
             # one instance a ring size - its fewest steps, at full length -
@@ -529,10 +539,10 @@ def rk4_section(dev, lib, args, threads, ref):
                 log(f"bench: {name} CPU {csteps / tc:.0f} steps/s; {bad} of {len(jobs)} differ")
             # on the sample: the same one launch against one batched launch a
             # ring size, bit for bit
-            sp = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=s2)
-                  for p, g, s2 in zip(progs, groups, samp)]
+            sp, _kw = apply_knobs([dict(prog=p, bank=g[2], state=g[4].copy(), steps=s2)
+                                   for p, g, s2 in zip(progs, groups, samp)], knobs)
             t0 = time.perf_counter()
-            run_multi(d2, sp, **knobs)
+            run_multi(d2, sp, **kw)
             t_multi = time.perf_counter() - t0
             eq = True
             t0 = time.perf_counter()
@@ -548,10 +558,10 @@ def rk4_section(dev, lib, args, threads, ref):
             # the plain form - one kernel launch an instruction, a batch a
             # ring size - beside the one launch, on steps // 2000 (at least 1)
             samp2 = [np.maximum(1, g[5] // 2000).astype(np.uint32) for g in groups]
-            sq = [dict(prog=p, bank=g[2], state=g[4].copy(), steps=s2)
-                  for p, g, s2 in zip(progs, groups, samp2)]
+            sq, _kw = apply_knobs([dict(prog=p, bank=g[2], state=g[4].copy(), steps=s2)
+                                   for p, g, s2 in zip(progs, groups, samp2)], knobs)
             t0 = time.perf_counter()
-            run_multi(d2, sq, **knobs)
+            run_multi(d2, sq, **kw)
             t_one = time.perf_counter() - t0
             t0 = time.perf_counter()
             nl, eq2 = 0, True

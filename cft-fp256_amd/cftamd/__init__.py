@@ -8,10 +8,10 @@ package binds it with ctypes, as the reference's own binding does.
 from . import _lib
 from ._lib import (CftError, FP32, FP64, FP128, FP256, RNE, RTZ, RDN, RUP, RMM,
                    check, lib, library_path)
-from .core import (Buffer, Device, DeviceArray, Program, check_image, last_plan, run_multi,
-                   tune_multi)
+from .core import (Buffer, Device, DeviceArray, Program, apply_knobs, check_image, last_plan,
+                   run_multi, tune_batch, tune_multi)
 
 __all__ = ["CftError", "Device", "Buffer", "DeviceArray", "Program", "check_image", "run_multi", "last_plan",
-           "tune_multi",
+           "tune_multi", "tune_batch", "apply_knobs",
            "FP32", "FP64", "FP128", "FP256", "RNE", "RTZ", "RDN", "RUP", "RMM", "check", "lib",
            "library_path", "_lib"]

@@ -281,22 +281,65 @@ def last_plan(dev):
 
 
 def tune_multi(dev, parts, trials=None):
-    """The autotuner: each planner setting in `trials` (dicts of run_multi's
-    knobs; by default lone-wave fractions 0.3..1.0 with the planner's own
-    waves a CU, and 4 and 8 waves a CU) runs on copies of the parts' states,
-    and the fastest is returned with every trial's seconds.  Run it on a
-    sample of the workload (steps scaled down): the planner's choices depend
-    on the ratios of the steps, not their size."""
+    """The autotuner for cftx_program_run_multi: each setting in `trials`
+    runs on copies of the parts' states and the fastest is returned with
+    every trial's seconds.  A setting is a dict of run_multi's knobs
+    (alone, waves_per_cu, waves_per_group) and optionally "width": lanes an
+    instance forced on every part (instance packing: 64 / width instances a
+    wave).  By default: the planner's lone-wave fraction 0.3..1.0; 4 and 8
+    waves a CU; workgroups of 1, 2 and 4 waves; and every part at 1, 2, 4,
+    8 and 16 lanes an instance.  Run it on a sample of the workload (steps
+    scaled down): the choices depend on the ratios of the steps, not their
+    size."""
     import time
     if trials is None:
         trials = [dict(alone=a) for a in (0.3, 0.5, 0.7, 0.9, 1.0)]
         trials += [dict(alone=a, waves_per_cu=k) for a in (0.5, 0.9) for k in (4, 8)]
+        trials += [dict(waves_per_group=g) for g in (1, 2, 4)]
+        trials += [dict(width=w) for w in (1, 2, 4, 8, 16)]
     results = []
     for t in trials:
+        knobs = {k: v for k, v in t.items() if k != "width"}
         trial = [dict(p, state=np.array(p["state"], copy=True)) for p in parts]
+        if "width" in t:
+            for q in trial:
+                q["lanes_per_instance"] = t["width"]
         try:
             t0 = time.perf_counter()
-            run_multi(dev, trial, **t)
+            run_multi(dev, trial, **knobs)
+            results.append((time.perf_counter() - t0, t))
+        except CftError:
+            continue
+    results.sort(key=lambda r: r[0])
+    return results[0][1], [(round(s, 5), t) for s, t in results]
+
+
+def apply_knobs(parts, knobs):
+    """run_multi's keyword arguments for a tune_multi setting, and the parts
+    with its forced width (if any) applied."""
+    kw = {k: v for k, v in knobs.items() if k != "width"}
+    if "width" in knobs:
+        parts = [dict(p, lanes_per_instance=knobs["width"]) for p in parts]
+    return parts, kw
+
+
+def tune_batch(prog, state, steps, bank=None, trials=None):
+    """The autotuner for one program's batched run (cftx_program_run_batch,
+    persistent): each setting - lanes an instance (1 interprets an instance
+    on one lane; 2..64 spread it, pure programs only; 0 the library's model)
+    and workgroup size in waves - runs on a copy of the state, the fastest
+    is returned with every trial's seconds.  Run it on a sample (steps
+    scaled down)."""
+    import time
+    if trials is None:
+        trials = [dict(lanes_per_instance=w, waves_per_group=g)
+                  for w in (0, 1, 2, 4, 8, 16) for g in (0, 1, 2, 4)]
+    results = []
+    for t in trials:
+        st = np.array(state, copy=True)
+        try:
+            t0 = time.perf_counter()
+            prog.run_batch(st, steps, bank=bank, **t)
             results.append((time.perf_counter() - t0, t))
         except CftError:
             continue
