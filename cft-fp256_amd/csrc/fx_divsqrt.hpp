@@ -143,7 +143,12 @@ __device__ __forceinline__ void div_op(const uint32_t (&xa)[NL], const uint32_t 
 #pragma unroll
     for (int i = 0; i < NL; i++)
         hi[i] = ua.m[i];
-    shl<NL>(hi, (uint32_t)(k + SD - 32 * NL));   // 2 or 3
+    // k + SD - 32NL is 2 or 3: the constant 3 (folded), then one bit back
+    // when ma >= mb (ma << 3 has its low bits clear)
+    shl<NL>(hi, 3u);
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        hi[i] = alignbit(i + 1 < NL ? hi[i + 1] : 0u, hi[i], ge ? 1u : 0u);
 #pragma unroll
     for (int i = 0; i < NL; i++) {
         U[i] = 0;
@@ -337,15 +342,20 @@ __device__ __forceinline__ void sqrt_op(const uint32_t (&xa)[NL], int rnd, uint3
         flags |= FL_INV;
         return;
     }
-    // M = m << t, top bit at 64NL-1 or 64NL-2, (e - t) even
+    // M = m << t, top bit at 64NL-1 or 64NL-2, (e - t) even: a shift by the
+    // constant 64NL - P (folded at compile time), then one bit back when t is
+    // one less (m << 64NL-P has its low bits clear, so nothing is lost)
     int t = 64 * NL - P;                         // top bit at 64NL - 1
-    if ((ua.e - t) & 1)
-        t--;
+    const uint32_t back = (uint32_t)((ua.e - t) & 1);
+    t -= (int)back;
     uint32_t M[2 * NL];
 #pragma unroll
     for (int i = 0; i < 2 * NL; i++)
         M[i] = i < NL ? ua.m[i] : 0u;
-    shl<2 * NL>(M, (uint32_t)t);
+    shl<2 * NL>(M, (uint32_t)(64 * NL - P));
+#pragma unroll
+    for (int i = 0; i < 2 * NL; i++)
+        M[i] = alignbit(i + 1 < 2 * NL ? M[i + 1] : 0u, M[i], back);
     uint32_t r[NL];
     bool rem_nz;
     isqrt_limbs<NL>(M, r, rem_nz);
