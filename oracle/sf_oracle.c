@@ -1018,7 +1018,7 @@ int oc_round_pack(int fmt, int sign, const uint8_t *m, int mbytes, int e,
 typedef struct {
     int fmt, op, rnd;
     const uint8_t *a, *b, *c;
-    uint8_t *d;
+    uint8_t *d, *lane_flags;
     size_t lo, hi;
     uint32_t smask, flags;
 } run_job;
@@ -1047,6 +1047,8 @@ static void *run_slice(void *arg)
             oc_compute(J->fmt, J->op, J->rnd, pa, pb, pc, tmp, &fl);
         }
         memcpy(J->d + i * esz, tmp, esz);
+        if (J->lane_flags)
+            J->lane_flags[i] = (uint8_t)fl;
         acc |= fl;
     }
     J->flags = acc;
@@ -1056,6 +1058,13 @@ static void *run_slice(void *arg)
 int oc_run(int fmt, int op, int rnd, const void *a, const void *b,
            const void *c, void *d, size_t n, uint32_t scalar_mask,
            int threads, uint32_t *flags_out)
+{
+    return oc_run_lanes(fmt, op, rnd, a, b, c, d, n, scalar_mask, threads, flags_out, NULL);
+}
+
+int oc_run_lanes(int fmt, int op, int rnd, const void *a, const void *b,
+                 const void *c, void *d, size_t n, uint32_t scalar_mask,
+                 int threads, uint32_t *flags_out, uint8_t *lane_flags)
 {
     run_job jobs[256];
     pthread_t tid[256];
@@ -1070,6 +1079,7 @@ int oc_run(int fmt, int op, int rnd, const void *a, const void *b,
         J->fmt = fmt; J->op = op; J->rnd = rnd;
         J->a = (const uint8_t *)a; J->b = (const uint8_t *)b;
         J->c = (const uint8_t *)c; J->d = (uint8_t *)d;
+        J->lane_flags = lane_flags;
         J->lo = n * (size_t)t / (size_t)threads;
         J->hi = n * (size_t)(t + 1) / (size_t)threads;
         J->smask = scalar_mask;

@@ -68,6 +68,12 @@ int oc_run(int fmt, int op, int rnd, const void *a, const void *b,
            const void *c, void *d, size_t n, uint32_t scalar_mask,
            int threads, uint32_t *flags_out);
 
+/* oc_run, and element i's own flags into lane_flags[i] (n bytes) where
+ * lane_flags is not NULL. */
+int oc_run_lanes(int fmt, int op, int rnd, const void *a, const void *b,
+                 const void *c, void *d, size_t n, uint32_t scalar_mask,
+                 int threads, uint32_t *flags_out, uint8_t *lane_flags);
+
 /* The reductions (rd_oracle.c): cft_reduce's one result of op (24 SUM,
  * 25 DOT, 28 SUMSQ, 29 SUMABS, 31 MAXALL) over n elements, and
  * cft_reduce_seg's n / seg results.  b is read by DOT only.  Returns 1 for

@@ -42,6 +42,8 @@ def oracle():
     o = ctypes.CDLL(ORACLE_SO)
     o.oc_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
                          ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int, U32P]
+    o.oc_run_lanes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
+                               ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int, U32P, V]
     o.oc_compute.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V, U32P]
     o.oc_div.argtypes = [ctypes.c_int, ctypes.c_int, V, V, V, U32P]
     o.oc_sqrt.argtypes = [ctypes.c_int, ctypes.c_int, V, V, U32P]
@@ -81,6 +83,8 @@ def emu():
     e = ctypes.CDLL(EMU_SO)
     e.emu_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
                           ctypes.c_size_t, U32P]
+    e.emu_hardcases.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                ctypes.c_size_t, V, V, V, V]
     return e
 
 

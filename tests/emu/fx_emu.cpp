@@ -4,6 +4,7 @@
 #include "fx_arith.hpp"
 #include "fx_divsqrt.hpp"
 #include "fx_misc.hpp"
+#include "hardcase.hpp"
 
 #include <string.h>
 
@@ -93,4 +94,43 @@ extern "C" size_t emu_cmp_add(int fmt, int rnd, int sub, const void *a, const vo
         }
     }
     return bad;
+}
+
+// The hard-case generator (hardcase.hpp), host build: cases base .. base+n-1,
+// the arrays laid out as cftp_hardcases (lib/libcftprobe.so) writes them on
+// the GPU, so the two can be compared bit for bit.
+template <int NL>
+static void hard(int op, uint64_t seed, uint64_t base, size_t n, uint32_t *a, uint32_t *b,
+                 uint32_t *c, uint8_t *fam)
+{
+    for (size_t i = 0; i < n; i++) {
+        uint32_t xa[NL], xb[NL], xc[NL];
+        const int f = hc::case_at<NL>(seed, base + i, op, xa, xb, xc);
+        memcpy(a + i * NL, xa, sizeof xa);
+        if (b)
+            memcpy(b + i * NL, xb, sizeof xb);
+        if (c)
+            memcpy(c + i * NL, xc, sizeof xc);
+        if (fam)
+            fam[i] = (uint8_t)f;
+    }
+}
+
+extern "C" int emu_hardcases(int fmt, int op, uint64_t seed, uint64_t base, size_t n, void *a,
+                             void *b, void *c, uint8_t *fam)
+{
+    if (fmt < 0 || fmt > 3 || !hc::op_known(op))
+        return -1;
+    if (!hc::op_reads_b(op))
+        b = nullptr;
+    if (!hc::op_reads_c(op))
+        c = nullptr;
+    uint32_t *A = (uint32_t *)a, *B = (uint32_t *)b, *C = (uint32_t *)c;
+    switch (fmt) {
+    case 0: hard<1>(op, seed, base, n, A, B, C, fam); break;
+    case 1: hard<2>(op, seed, base, n, A, B, C, fam); break;
+    case 2: hard<4>(op, seed, base, n, A, B, C, fam); break;
+    default: hard<8>(op, seed, base, n, A, B, C, fam); break;
+    }
+    return 0;
 }
