@@ -198,16 +198,45 @@ __device__ __forceinline__ void product(const uint32_t (&a)[NL], const uint32_t 
     fma_core<NL>(OP_MUL, a, b, z, RNE, p, fl);
 }
 
+// Every draw below is a statement of its own (or the one draw of an
+// expression): the order of evaluation of a call's arguments is unspecified
+// in C++, and host and device compilers do choose differently, so two draws
+// in one argument list would make the CPU and GPU builds different cases.
+
+// A random sign, then value().
+template <int NL>
+__device__ __forceinline__ void rvalue(Rng &r, int E, int kind, uint32_t (&x)[NL])
+{
+    const int s = r.bit();
+    value<NL>(r, s, E, kind, x);
+}
+
+__device__ __forceinline__ int any_kind(Rng &r)
+{
+    const int k = r.in(0, 7);
+    return k <= 2 ? K_RAND : k - 2;               // random three times in eight
+}
+
 // a and b whose product leads at about Et (ea + eb = Et; the product of two
 // significands adds 0 or 1), spread so neither is extreme.
 template <int NL>
-__device__ __forceinline__ void pair_at(Rng &r, int Et, int ka, int kb, uint32_t (&a)[NL], uint32_t (&b)[NL])
+__device__ __forceinline__ void pair_at(Rng &r, int Et, int ka, int kb, uint32_t (&a)[NL],
+                                        uint32_t (&b)[NL])
 {
     using Fm = F<NL>;
     constexpr int spread = Fm::EMAX / 2 < 1000 ? Fm::EMAX / 2 : 1000;
     const int ea = Et / 2 + r.in(-spread, spread);
-    value<NL>(r, r.bit(), ea, ka, a);
-    value<NL>(r, r.bit(), Et - ea, kb, b);
+    rvalue<NL>(r, ea, ka, a);
+    rvalue<NL>(r, Et - ea, kb, b);
+}
+
+// pair_at with both kinds drawn by any_kind, a's first.
+template <int NL>
+__device__ __forceinline__ void pair_any(Rng &r, int Et, uint32_t (&a)[NL], uint32_t (&b)[NL])
+{
+    const int ka = any_kind(r);
+    const int kb = any_kind(r);
+    pair_at<NL>(r, Et, ka, kb, a, b);
 }
 
 template <int NL>
@@ -219,8 +248,6 @@ __device__ __forceinline__ void signed_zero(int s, uint32_t (&x)[NL])
     x[NL - 1] = (uint32_t)(s & 1) << 31;
 }
 
-__device__ __forceinline__ int any_kind(Rng &r);
-
 // An operand at the subnormal edge: subnormal, just above EMIN, a few bits
 // above the smallest subnormal, or near 1 (so a product with a subnormal
 // stays at the edge).
@@ -228,18 +255,16 @@ template <int NL>
 __device__ __forceinline__ void edge_operand(Rng &r, uint32_t (&x)[NL])
 {
     using Fm = F<NL>;
-    switch (r.in(0, 3)) {
-    case 0: value<NL>(r, r.bit(), Fm::EMIN - r.in(1, Fm::P), any_kind(r), x); break;
-    case 1: value<NL>(r, r.bit(), Fm::EMIN + r.in(0, 2), any_kind(r), x); break;
-    case 2: value<NL>(r, r.bit(), Fm::EMIN - Fm::P + r.in(1, 8), K_RAND, x); break;
-    default: value<NL>(r, r.bit(), r.in(-Fm::P, Fm::P), any_kind(r), x); break;
+    const int w = r.in(0, 3);
+    int E;
+    switch (w) {
+    case 0: E = Fm::EMIN - r.in(1, Fm::P); break;
+    case 1: E = Fm::EMIN + r.in(0, 2); break;
+    case 2: E = Fm::EMIN - Fm::P + r.in(1, 8); break;
+    default: E = r.in(-Fm::P, Fm::P); break;
     }
-}
-
-__device__ __forceinline__ int any_kind(Rng &r)
-{
-    const int k = r.in(0, 7);
-    return k <= 2 ? K_RAND : k - 2;               // random three times in eight
+    const int k = w == 2 ? (int)K_RAND : any_kind(r);
+    rvalue<NL>(r, E, k, x);
 }
 
 template <int NL>
@@ -247,7 +272,8 @@ __device__ __forceinline__ void special(Rng &r, uint32_t (&x)[NL])
 {
     using Fm = F<NL>;
     const int s = r.bit();
-    switch (r.in(0, 13)) {
+    const int w = r.in(0, 13);
+    switch (w) {
     case 0: signed_zero<NL>(s, x); break;
     case 1: case 2:
 #pragma unroll
@@ -268,71 +294,95 @@ __device__ __forceinline__ void special(Rng &r, uint32_t (&x)[NL])
     case 9: value<NL>(r, s, 0, K_ZERO, x); break;                     // +-1
     case 10: value<NL>(r, s, 0, K_LOW, x); break;                     // 1 + ulp
     case 11: value<NL>(r, s, -1, K_ONES, x); break;                   // 1 - ulp/2
-    case 12: value<NL>(r, s, r.in(-64, 64), K_RAND, x); break;
+    case 12: {
+        const int E = r.in(-64, 64);
+        value<NL>(r, s, E, K_RAND, x);
+        break;
+    }
     default: raw<NL>(r, x); break;
     }
 }
 
 template <int NL>
-__device__ __forceinline__ void make_case(Rng &r, int fam, int op, uint32_t (&a)[NL], uint32_t (&b)[NL],
-                          uint32_t (&c)[NL])
+__device__ __forceinline__ void make_case(Rng &r, int fam, int op, uint32_t (&a)[NL],
+                                          uint32_t (&b)[NL], uint32_t (&c)[NL])
 {
     using Fm = F<NL>;
     constexpr int P = Fm::P, EMIN = Fm::EMIN, EMAX = Fm::EMAX;
     uint32_t p[NL];
     bool have_p = false;
-    auto mid = [&](Rng &q) { return q.in(EMIN / 2, EMAX / 2); };
     switch (fam) {
     case 0:
-        raw<NL>(r, a); raw<NL>(r, b); raw<NL>(r, c);
+        raw<NL>(r, a);
+        raw<NL>(r, b);
+        raw<NL>(r, c);
         break;
     case 1: {                                     // operands at the subnormal edge
         edge_operand<NL>(r, a);
         edge_operand<NL>(r, b);
-        const int k = r.in(0, 3);
-        if (k == 3)
-            signed_zero<NL>(r.bit(), c);
-        else
-            value<NL>(r, r.bit(), k == 0 ? EMIN - r.in(0, P) : EMIN + r.in(0, 2 * P),
-                      any_kind(r), c);
+        const int w = r.in(0, 3);
+        if (w == 3) {
+            const int s = r.bit();
+            signed_zero<NL>(s, c);
+        } else {
+            const int E = w == 0 ? EMIN - r.in(0, P) : EMIN + r.in(0, 2 * P);
+            const int k = any_kind(r);
+            rvalue<NL>(r, E, k, c);
+        }
         break;
     }
     case 2: {                                     // product at the underflow edge
         const int Et = EMIN + r.in(-(P + 3), 3);
-        pair_at<NL>(r, Et, any_kind(r), any_kind(r), a, b);
-        switch (r.in(0, 3)) {
-        case 0: signed_zero<NL>(r.bit(), c); break;
-        case 1: value<NL>(r, r.bit(), EMIN - r.in(1, P), any_kind(r), c); break;
-        case 2: value<NL>(r, r.bit(), Et + r.in(-3, 3), any_kind(r), c); break;
-        default: value<NL>(r, r.bit(), EMIN, K_ZERO, c); add_ulps<NL>(c, r.in(-3, 3)); break;
+        pair_any<NL>(r, Et, a, b);
+        const int w = r.in(0, 3);
+        if (w == 0) {
+            const int s = r.bit();
+            signed_zero<NL>(s, c);
+        } else if (w == 3) {
+            rvalue<NL>(r, EMIN, K_ZERO, c);
+            const int u = r.in(-3, 3);
+            add_ulps<NL>(c, u);
+        } else {
+            const int E = w == 1 ? EMIN - r.in(1, P) : Et + r.in(-3, 3);
+            const int k = any_kind(r);
+            rvalue<NL>(r, E, k, c);
         }
         break;
     }
     case 3: {                                     // product at the overflow edge
         const int Et = EMAX + r.in(-2, 1);
-        pair_at<NL>(r, Et, r.bit() ? K_ONES : K_RAND, any_kind(r), a, b);
-        switch (r.in(0, 3)) {
-        case 0: signed_zero<NL>(r.bit(), c); break;
-        case 1: value<NL>(r, r.bit(), EMAX - r.in(0, P + 2), any_kind(r), c); break;
-        case 2: value<NL>(r, r.bit(), EMAX, K_ONES, c); break;
-        default: value<NL>(r, r.bit(), r.in(-10, 10), K_RAND, c); break;
+        const int ka = r.bit() ? K_ONES : K_RAND;
+        const int kb = any_kind(r);
+        pair_at<NL>(r, Et, ka, kb, a, b);
+        const int w = r.in(0, 3);
+        if (w == 0) {
+            const int s = r.bit();
+            signed_zero<NL>(s, c);
+        } else if (w == 2) {
+            rvalue<NL>(r, EMAX, K_ONES, c);
+        } else {
+            const int E = w == 1 ? EMAX - r.in(0, P + 2) : r.in(-10, 10);
+            const int k = w == 1 ? any_kind(r) : (int)K_RAND;
+            rvalue<NL>(r, E, k, c);
         }
         break;
     }
     case 4: case 9: case 10: {                    // cancellation: c = -(a*b rounded) +- k ulps
-        const int Et = fam == 4 ? mid(r) : fam == 9 ? EMIN + r.in(-4, P + 4) : EMAX - r.in(1, 4);
-        pair_at<NL>(r, Et, any_kind(r), any_kind(r), a, b);
+        const int Et = fam == 4 ? r.in(EMIN / 2, EMAX / 2)
+                     : fam == 9 ? EMIN + r.in(-4, P + 4) : EMAX - r.in(1, 4);
+        pair_any<NL>(r, Et, a, b);
         product<NL>(a, b, p);
         have_p = true;
         copy<NL>(p, c);
         neg<NL>(c);
-        add_ulps<NL>(c, r.in(-8, 8));
+        const int u = r.in(-8, 8);
+        add_ulps<NL>(c, u);
         break;
     }
     case 5: {                                     // exact cancellation / the product's error
         const int Et = r.in(EMIN + 2 * P, EMAX - 2);
-        const bool exact = r.bit();
-        pair_at<NL>(r, Et, exact ? K_SHORT : K_RAND, exact ? K_SHORT : K_RAND, a, b);
+        const int k = r.bit() ? K_SHORT : K_RAND;
+        pair_at<NL>(r, Et, k, k, a, b);
         product<NL>(a, b, p);
         have_p = true;
         copy<NL>(p, c);
@@ -340,31 +390,41 @@ __device__ __forceinline__ void make_case(Rng &r, int fam, int op, uint32_t (&a)
         break;
     }
     case 6: {                                     // ties of an exact product and c
-        pair_at<NL>(r, mid(r), K_SHORT, K_SHORT, a, b);
+        const int Et = r.in(EMIN / 2, EMAX / 2);
+        pair_at<NL>(r, Et, K_SHORT, K_SHORT, a, b);
         product<NL>(a, b, p);
         have_p = true;
         const int Ep = lead_exp<NL>(p);
-        switch (r.in(0, 3)) {
-        case 0: value<NL>(r, r.bit(), Ep - P, K_ZERO, c); break;        // half an ulp
-        case 1: value<NL>(r, r.bit(), Ep - P, K_LOW, c); break;         // just above
-        case 2: value<NL>(r, r.bit(), Ep - P - 1, K_ONES, c); break;    // just below
-        default: value<NL>(r, r.bit(), Ep - P, K_HALF, c); break;       // three quarters
+        const int w = r.in(0, 3);
+        switch (w) {
+        case 0: rvalue<NL>(r, Ep - P, K_ZERO, c); break;        // half an ulp
+        case 1: rvalue<NL>(r, Ep - P, K_LOW, c); break;         // just above
+        case 2: rvalue<NL>(r, Ep - P - 1, K_ONES, c); break;    // just below
+        default: rvalue<NL>(r, Ep - P, K_HALF, c); break;       // three quarters
         }
         break;
     }
     case 7: {                                     // c swept across the product's binades
         const int Et = r.in(EMIN / 4, EMAX / 4);
-        pair_at<NL>(r, Et, any_kind(r), any_kind(r), a, b);
-        value<NL>(r, r.bit(), Et + r.in(-(2 * P + 40), 2 * P + 40), any_kind(r), c);
+        pair_any<NL>(r, Et, a, b);
+        const int E = Et + r.in(-(2 * P + 40), 2 * P + 40);
+        const int k = any_kind(r);
+        rvalue<NL>(r, E, k, c);
         break;
     }
     case 8:
-        special<NL>(r, a); special<NL>(r, b); special<NL>(r, c);
+        special<NL>(r, a);
+        special<NL>(r, b);
+        special<NL>(r, c);
         break;
     case 11: {                                    // c at the normal boundary
-        value<NL>(r, r.bit(), r.bit() ? EMIN : EMIN - 1, r.bit() ? K_ZERO : K_ONES, c);
-        add_ulps<NL>(c, r.in(-3, 3));
-        pair_at<NL>(r, r.bit() ? EMIN + r.in(-3, 3) : mid(r), any_kind(r), any_kind(r), a, b);
+        const int E = r.bit() ? EMIN : EMIN - 1;
+        const int k = r.bit() ? K_ZERO : K_ONES;
+        rvalue<NL>(r, E, k, c);
+        const int u = r.in(-3, 3);
+        add_ulps<NL>(c, u);
+        const int Et = r.bit() ? EMIN + r.in(-3, 3) : r.in(EMIN / 2, EMAX / 2);
+        pair_any<NL>(r, Et, a, b);
         break;
     }
     case 12: {                                    // overflow by a rounding carry
@@ -374,23 +434,27 @@ __device__ __forceinline__ void make_case(Rng &r, int fam, int op, uint32_t (&a)
             pair_at<NL>(r, Ep - 1, K_ONES, K_ONES, a, b);
         } else {                                  // the largest finite times ~1
             Ep = EMAX;
-            value<NL>(r, r.bit(), EMAX, K_ONES, a);
-            const int k = r.in(0, 2);
-            value<NL>(r, r.bit(), k == 2 ? -1 : 0, k == 0 ? K_ZERO : k == 1 ? K_LOW : K_ONES, b);
+            rvalue<NL>(r, EMAX, K_ONES, a);
+            const int w = r.in(0, 2);
+            rvalue<NL>(r, w == 2 ? -1 : 0, w == 0 ? K_ZERO : w == 1 ? K_LOW : K_ONES, b);
         }
-        const int k = r.in(0, 2);
-        value<NL>(r, r.in(0, 3) ? ((a[NL - 1] ^ b[NL - 1]) >> 31) : r.bit(),
-                  k == 0 ? Ep - P - 1 : Ep - P, k == 0 ? K_ONES : k == 1 ? K_ZERO : K_LOW, c);
+        const int w = r.in(0, 2);
+        // c pushes the product up (its sign) three times in four
+        const int s = r.in(0, 3) ? (int)((a[NL - 1] ^ b[NL - 1]) >> 31) : r.bit();
+        value<NL>(r, s, w == 0 ? Ep - P - 1 : Ep - P, w == 0 ? K_ONES : w == 1 ? K_ZERO : K_LOW,
+                  c);
         break;
     }
     default: {                                    // 13: alignment edges
-        pair_at<NL>(r, r.in(EMIN / 4, EMAX / 4), any_kind(r), any_kind(r), a, b);
+        const int Et = r.in(EMIN / 4, EMAX / 4);
+        pair_any<NL>(r, Et, a, b);
         product<NL>(a, b, p);
         have_p = true;
         const int Ep = lead_exp<NL>(p);
         const int s = r.bit() ? 1 : -1;
+        const int w = r.in(0, 7);
         int d;
-        switch (r.in(0, 7)) {
+        switch (w) {
         case 0: d = r.in(31, 43); break;
         case 1: d = -(P + r.in(31, 43)); break;
         case 2: d = s * (2 * P + r.in(-4, 8)); break;
@@ -401,8 +465,7 @@ __device__ __forceinline__ void make_case(Rng &r, int fam, int op, uint32_t (&a)
         default: d = s * (Fm::T - 2 * P + r.in(-4, 4)); break;
         }
         const int k = r.in(0, 3);
-        value<NL>(r, r.bit(), Ep + d, k == 0 ? K_RAND : k == 1 ? K_ONES : k == 2 ? K_ZERO : K_LOW,
-                  c);
+        rvalue<NL>(r, Ep + d, k == 0 ? K_RAND : k == 1 ? K_ONES : k == 2 ? K_ZERO : K_LOW, c);
         break;
     }
     }
@@ -414,7 +477,8 @@ __device__ __forceinline__ void make_case(Rng &r, int fam, int op, uint32_t (&a)
             if (!have_p)
                 product<NL>(a, b, p);
             if (op == G_DIV) {
-                add_ulps<NL>(p, r.in(-2, 2));
+                const int u = r.in(-2, 2);
+                add_ulps<NL>(p, u);
                 copy<NL>(p, a);
             } else {
                 copy<NL>(p, a);
@@ -425,7 +489,8 @@ __device__ __forceinline__ void make_case(Rng &r, int fam, int op, uint32_t (&a)
     } else if (op == G_SQRT && !rawfam) {
         uint32_t q[NL];
         product<NL>(a, a, q);
-        add_ulps<NL>(q, r.in(-2, 2));
+        const int u = r.in(-2, 2);
+        add_ulps<NL>(q, u);
         copy<NL>(q, a);
     }
 }

@@ -144,3 +144,31 @@ def test_reference_agrees_on_hard_cases(emu, oracle, ref, fmt):
                 assert f.value == fl
     finally:
         ref.cft_close(dev)
+
+
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+
+def test_generator_same_under_two_compilers(emu, tmp_path):
+    """The order of evaluation of a call's arguments is unspecified, and g++
+    and clang (the GPU build's compiler) choose differently: the generator
+    must make the same cases under both, so a second host build with clang
+    is held to the first."""
+    import os
+    import subprocess
+    from conftest import EMU_DIR, PKG
+    if not os.path.exists(CLANG):
+        pytest.skip("no clang++ here")
+    so = str(tmp_path / "libfxemu_clang.so")
+    subprocess.run([CLANG, "-O2", "-std=c++17", "-shared", "-fPIC", "-Wno-unknown-pragmas",
+                    "-Wno-pass-failed",
+                    "-I", EMU_DIR, "-I", os.path.join(PKG, "csrc"),
+                    os.path.join(EMU_DIR, "fx_emu.cpp"), "-o", so], check=True)
+    other = ctypes.CDLL(so)
+    other.emu_hardcases.argtypes = emu.emu_hardcases.argtypes
+    for fmt in range(4):
+        for op in OPS.values():
+            x = gen(emu, fmt, op, 31, 77, 14 * 50)
+            y = gen(other, fmt, op, 31, 77, 14 * 50)
+            for u, v in zip(x, y):
+                assert (u == v).all(), (fmt, op)
