@@ -278,6 +278,13 @@ cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io
     const void *dbank = io.bank, *da = io.a, *db = io.b, *dc = io.c;
     void *dstate = io.state, *dlflags = io.lflags;
     const void *dorder = nullptr, *dsteps = nullptr;
+    // the batch takes its longest lanes' time (order is longest first): the
+    // waves holding lanes of at least half as many trips issue first
+    uint32_t hi_steps = 0;
+    if (n > 1 && io.steps && io.order) {
+        const uint32_t most = io.steps[io.order[0]];
+        hi_steps = most > 1 ? (most + 1) / 2 : 0;
+    }
     {
         cft_status bs = bind_in(ex, ROLE_ORDER, io.order, n * 4, &dorder, who);
         if (bs == CFT_OK)
@@ -366,6 +373,7 @@ cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io
         B.ntiles = (uint32_t)ntiles;
         B.lpw = ipw;
         B.seg = seg;
+        B.hi_steps = hi_steps;
         SeqWide Wd;
         memset(&Wd, 0, sizeof Wd);
         Wd.recs = reinterpret_cast<const uint4 *>(wd->d_recs);
@@ -392,7 +400,7 @@ cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io
                                       (uint64_t)(uintptr_t)dc, (uint64_t)(uintptr_t)dlflags,
                                       (uint64_t)(uintptr_t)S.pool, n, ipw, (uint64_t)waves,
                                       blocks, npool, seg, reach, 2, (uint64_t)width,
-                                      (uint64_t)(uintptr_t)wd->d_recs, 0, 0, 0, 0, 0};
+                                      (uint64_t)(uintptr_t)wd->d_recs, hi_steps, 0, 0, 0, 0};
             if (!prog->gexec || memcmp(key, prog->gkey, sizeof key) != 0) {
                 seq_drop_graph(prog);
                 e = hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal);
@@ -468,6 +476,7 @@ cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io
         B.ntiles = (uint32_t)ntiles;
         B.lpw = lpw;
         B.seg = seg;
+        B.hi_steps = hi_steps;
         auto record = [&]() -> hipError_t {
             hipError_t r = cftk_fill32(ex->dwords, 0u, 3, ex->stream);
             if (r == hipSuccess)
@@ -483,7 +492,7 @@ cft_status batch_persistent(cft_device *ex, cft_program *prog, const BatchIO &io
                                       (uint64_t)(uintptr_t)da, (uint64_t)(uintptr_t)db,
                                       (uint64_t)(uintptr_t)dc, (uint64_t)(uintptr_t)dlflags,
                                       (uint64_t)(uintptr_t)S.pool, n, lpw, (uint64_t)sh.waves, blocks,
-                                      npool, seg, reach, 1, 0, 0, 0, 0, 0, 0, 0};
+                                      npool, seg, reach, 1, hi_steps, 0, 0, 0, 0, 0, 0};
             if (!prog->gexec || memcmp(key, prog->gkey, sizeof key) != 0) {
                 seq_drop_graph(prog);
                 e = hipStreamBeginCapture(ex->stream, hipStreamCaptureModeThreadLocal);

@@ -504,6 +504,22 @@ __global__ __launch_bounds__(256) void k_seq(SeqArgs A, uint32_t nregs, int phys
 // program (fresh registers and scratch, its state as the scratch-in block,
 // the scratch-out block back into its state) with the segment REPEAT at
 // min(steps left, seg) trips.
+// A persistent runner takes its tiles longest first, and the batch's time is
+// its longest lanes' time: a wave whose tile holds a lane of at least
+// hi_steps trips issues ahead of the other waves on its SIMD (they fill the
+// cycles it leaves), so the longest lanes run near their lone speed while
+// the rest of the batch shares the machine.  hi_steps 0 leaves every wave at
+// the default priority.
+__device__ __forceinline__ void tile_priority(uint32_t hi_steps, uint32_t left)
+{
+    if (hi_steps == 0)
+        return;
+    if (wave_any(left >= hi_steps))
+        __builtin_amdgcn_s_setprio(2);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
 template <int NL>
 __global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32_t nregs,
                                                    int phys_a, int phys_b, int phys_c,
@@ -531,6 +547,7 @@ __global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32
         const bool valid = lane < B.lpw && idx < B.n;
         const uint32_t id = valid ? B.order[idx] : 0u;
         uint32_t left = valid ? B.steps[id] : 0u;
+        tile_priority(B.hi_steps, left);
         uint32_t *st = B.state + (size_t)id * A.n_in * NL;
         uint32_t lfl = 0;
         while (wave_any(left > 0)) {
@@ -725,12 +742,25 @@ __device__ __forceinline__ uint32_t wide_run(const WideInst<NL> &I, const bool v
                 if (kind == WK_ALU) {
                     uint32_t xa[NL], xb[NL], xc[NL], o[NL], f = 0;
                     const uint32_t ia = WR_A(rec.y), ib = WR_B(rec.y), ic = WR_C(rec.z);
-                    if (WR_KA(rec.x)) row_get<NL>(I.konst + ia * NQ, 1, xa);
-                    else row_get<NL>(I.regs + ia * rstride, 1, xa);
-                    if (WR_KB(rec.x)) row_get<NL>(I.konst + ib * NQ, 1, xb);
-                    else row_get<NL>(I.regs + ib * rstride, 1, xb);
-                    if (WR_KC(rec.x)) row_get<NL>(I.konst + ic * NQ, 1, xc);
-                    else row_get<NL>(I.regs + ic * rstride, 1, xc);
+                    // constants and registers are rows of one LDS array: an
+                    // operand's row is an offset from the constants, chosen by
+                    // a mask (no branch), and one load
+                    const uint32_t rbase = (uint32_t)(I.regs - I.konst);
+                    auto row = [&](uint32_t idx, uint32_t k) -> uint32_t {
+                        const uint32_t ko = idx * (uint32_t)NQ;
+                        const uint32_t ro = rbase + idx * (uint32_t)rstride;
+                        return ro ^ ((ro ^ ko) & (0u - k));
+                    };
+                    row_get<NL>(I.konst + row(ia, WR_KA(rec.x)), 1, xa);
+                    row_get<NL>(I.konst + row(ib, WR_KB(rec.x)), 1, xb);
+                    row_get<NL>(I.konst + row(ic, WR_KC(rec.x)), 1, xc);
+                    {
+                        // fused ABS / NEG of an operand: clear, then flip, its sign
+                        const uint32_t m = WR_MODS(rec.x);
+                        xa[NL - 1] = (xa[NL - 1] & ~(m << 31)) ^ ((m >> 1) << 31);
+                        xb[NL - 1] = (xb[NL - 1] & ~((m >> 2) << 31)) ^ ((m >> 3) << 31);
+                        xc[NL - 1] = (xc[NL - 1] & ~((m >> 4) << 31)) ^ ((m >> 5) << 31);
+                    }
                     const uint32_t op = WR_OP(rec.x);
                     const bool addsub = op == OP_ADD || op == OP_SUB;
                     if (__builtin_amdgcn_ballot_w64(!addsub) == 0) {
@@ -870,6 +900,7 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
         const bool valid = idx < B.n;
         const uint32_t id = valid ? B.order[idx] : 0u;
         const uint32_t left = valid ? B.steps[id] : 0u;
+        tile_priority(B.hi_steps, left);
         uint32_t *st = B.state + (size_t)id * A.n_in * NL;
         const uint32_t lfl = recs_lds ? wide_run<NL, true>(I, valid, left, st, src, phys, id)
                                       : wide_run<NL, false>(I, valid, left, st, src, phys, id);

@@ -92,7 +92,9 @@ struct SeqBatch {
     uint32_t ntiles;
     uint32_t lpw;                     // lanes a wave takes (1..64)
     uint32_t seg;                     // trips a segment
-    uint32_t pad;
+    uint32_t hi_steps;                // a wave whose tile holds a lane with at
+                                      // least this many trips issues at raised
+                                      // priority (0: every wave at the default)
 };
 
 // The wide batched runner's program (seq_wide.hpp): `nsteps` steps of

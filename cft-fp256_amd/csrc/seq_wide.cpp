@@ -160,6 +160,7 @@ struct Node {
     int kind = WK_ALU;
     uint32_t op = 0, rnd = 0;
     Val src[3];
+    uint32_t mod[3] = {0, 0, 0};  // sign modifier on load: bit 0 clear, bit 1 flip
     uint32_t slot = 0;          // LOAD/STORE
     int home = -1;              // writes this home register directly (ALU, LOAD, MOVE)
     std::vector<int> raw, war;  // predecessors
@@ -174,6 +175,98 @@ struct BlockOut {
     std::vector<Node> nodes;
     uint32_t temps = 0;
 };
+
+// ABS and NEG (host/src/softfloat.c:954-961) change the sign bit alone and
+// raise no flag: as a sign modifier on a reader's operand they give it the
+// same bits.  A modifier is (clear, flip): x -> (clear ? |x| : x) ^ flip.
+uint32_t sign_mod_of(uint32_t op)
+{
+    return op == 4 ? 1u : op == 5 ? 2u : 0u;
+}
+
+// outer(inner(x)) as one modifier
+uint32_t compose_mod(uint32_t outer, uint32_t inner)
+{
+    if (outer & 1u)
+        return 1u | (outer & 2u);
+    return (inner & 1u) | ((inner ^ outer) & 2u);
+}
+
+// An ALU operand that is the result of an ABS or NEG in the block reads that
+// instruction's own operand, with the sign change as a modifier on load: the
+// ABS/NEG leaves the dependence chain, and is dropped when nothing else
+// (a store, a register live after the block, a reader that is not an ALU
+// operation) needs its result.  Nodes are in creation order, so a node's
+// sources are final before its readers are rewritten.
+void fuse_sign_ops(std::vector<Node> &N, std::map<uint32_t, Val> &stored,
+                   std::map<uint32_t, int> &entry_load, std::vector<Val> &cur,
+                   const VarSet &live_out, uint32_t nregs)
+{
+    // A register whose home is written in the block: a reader of its entry
+    // value must come before that write (a WAR edge).  An ABS/NEG of such an
+    // entry value stays, so no reader that depends on the write is made to
+    // read the entry value (which would order it both before and after it).
+    std::vector<bool> rewritten(32, false);
+    for (uint32_t r = 0; r < nregs; r++)
+        rewritten[r] = live_out[r] && !(cur[r] == Val{3, r});
+    for (size_t i = 0; i < N.size(); i++) {
+        if (N[i].kind != WK_ALU)
+            continue;
+        for (int k = 0; k < 3; k++) {
+            const Val v = N[i].src[k];
+            if (v.kind != 2)
+                continue;
+            const Node &q = N[v.idx];
+            const uint32_t m = q.kind == WK_ALU ? sign_mod_of(q.op) : 0u;
+            if (!m || (q.src[0].kind == 3 && rewritten[q.src[0].idx]))
+                continue;
+            N[i].src[k] = q.src[0];
+            N[i].mod[k] = compose_mod(compose_mod(N[i].mod[k], m), q.mod[0]);
+        }
+    }
+    std::vector<int> refs(N.size(), 0);
+    for (const Node &n : N)
+        for (int k = 0; k < 3; k++)
+            if (n.src[k].kind == 2)
+                refs[n.src[k].idx]++;
+    for (auto &kv : stored)
+        if (kv.second.kind == 2)
+            refs[kv.second.idx]++;
+    for (uint32_t r = 0; r < nregs; r++)
+        if (live_out[r] && cur[r].kind == 2)
+            refs[cur[r].idx]++;
+    std::vector<int> remap(N.size(), -1);
+    size_t kept = 0;
+    for (size_t i = 0; i < N.size(); i++) {
+        const bool dead = N[i].kind == WK_ALU && sign_mod_of(N[i].op) && refs[i] == 0;
+        if (!dead)
+            remap[i] = (int)kept++;
+    }
+    if (kept == N.size())
+        return;
+    std::vector<Node> M;
+    M.reserve(kept);
+    for (size_t i = 0; i < N.size(); i++)
+        if (remap[i] >= 0)
+            M.push_back(N[i]);
+    auto fix = [&](Val &v) {
+        if (v.kind == 2)
+            v.idx = (uint32_t)remap[v.idx];
+    };
+    for (Node &n : M)
+        for (int k = 0; k < 3; k++)
+            fix(n.src[k]);
+    for (auto &kv : stored)
+        fix(kv.second);
+    for (auto &kv : entry_load)
+        kv.second = remap[kv.second];
+    for (Val &v : cur)
+        if (v.kind == 2 && remap[v.idx] < 0)
+            v = Val{};                      // a dead value no live register holds
+        else
+            fix(v);
+    N.swap(M);
+}
 
 int weight(const Node &n)
 {
@@ -239,6 +332,7 @@ bool compile_block(const cft_program *p, uint32_t lo, uint32_t hi, const VarSet 
             stored[slot] = cur[SEQ_SRC_A(in.src)];
         }
     }
+    fuse_sign_ops(N, stored, entry_load, cur, live_out, nregs);
     // the stores and register values live after the block
     for (auto &kv : stored) {
         if (!live_out[var_slot(kv.first)])
@@ -454,7 +548,8 @@ struct Emitter {
                     switch (n.kind) {
                     case WK_ALU:
                         r.x = n.op | ((uint32_t)WK_ALU << 8) | (n.rnd << 11) | (ka << 14) |
-                              (kb << 15) | (kc << 16);
+                              (kb << 15) | (kc << 16) | (n.mod[0] << 17) | (n.mod[1] << 19) |
+                              (n.mod[2] << 21);
                         r.y = a | (b << 16);
                         r.z = c | ((uint32_t)n.loc << 16);
                         if (alu) (*alu)++;

@@ -13,8 +13,11 @@
 // stores to later loads of the same slot, loads each slot's entry value
 // once, keeps only the stores and register values that are live after the
 // block (liveness over the loop structure, the scratch-out block live at the
-// end), and list-schedules by critical path.  Every ALU instruction is kept,
-// since its flags are observable.  Each result is the same correctly rounded
+// end), and list-schedules by critical path.  Every ALU instruction that can
+// raise a flag is kept, since its flags are observable; ABS and NEG, which
+// change a sign bit and raise none, become sign modifiers on their readers'
+// operands (dropped when nothing else reads their result), which takes them
+// off the dependence chain.  Each result is the same correctly rounded
 // operation on the same operands, so the values, and the OR of the flags,
 // are those of the source program in source order.
 //
@@ -29,7 +32,9 @@
 
 // a step's record for one lane position: 16 bytes
 struct WideRec {
-    uint32_t x;   // [7:0] op, [10:8] kind, [13:11] rnd, [14] ka, [15] kb, [16] kc
+    uint32_t x;   // [7:0] op, [10:8] kind, [13:11] rnd, [14] ka, [15] kb, [16] kc,
+                  // [18:17] [20:19] [22:21] sign modifiers of a, b, c (bit 0 clear
+                  // the sign, bit 1 flip it, in that order: a fused ABS / NEG)
     uint32_t y;   // [15:0] a, [31:16] b
     uint32_t z;   // [15:0] c, [31:16] dst
     uint32_t w;   // control: REPEAT trips; unused otherwise
@@ -51,6 +56,7 @@ enum {
 #define WR_KA(x)   (((x) >> 14) & 1u)
 #define WR_KB(x)   (((x) >> 15) & 1u)
 #define WR_KC(x)   (((x) >> 16) & 1u)
+#define WR_MODS(x) (((x) >> 17) & 0x3Fu)
 #define WR_A(y)    ((y) & 0xFFFFu)
 #define WR_B(y)    ((y) >> 16)
 #define WR_C(z)    ((z) & 0xFFFFu)

@@ -1,6 +1,7 @@
 """The wide scheduler on the CPU: cftx_program_wide's step records, executed
 by a plain simulation of the wide kernel's semantics (one instance, its
-lanes' records in any order, the oracle computing each ALU operation), give
+lanes' records in any order, operands' sign modifiers applied on load, the
+oracle computing each ALU operation), give
 the same scratch-out block and flags as the oracle's ordinary run of the
 program - for the ODE programs at every width and for random pure
 programs."""
@@ -78,11 +79,16 @@ def simulate(oracle, recs, info, fmt, bank, sin, n_out, trips, a=None, phys_a=-1
             if kind == WK_NOP:
                 continue
             if kind == WK_ALU:
-                def src(v, k):
-                    return konst[v] if k else regs[v]
-                sa = src(y & 0xFFFF, (x >> 14) & 1)
-                sb = src(y >> 16, (x >> 15) & 1)
-                sc = src(z & 0xFFFF, (x >> 16) & 1)
+                def src(v, k, m):
+                    e = bytearray(konst[v] if k else regs[v])
+                    if m & 1:                            # a fused ABS
+                        e[-1] &= 0x7F
+                    if m & 2:                            # a fused NEG
+                        e[-1] ^= 0x80
+                    return bytes(e)
+                sa = src(y & 0xFFFF, (x >> 14) & 1, (x >> 17) & 3)
+                sb = src(y >> 16, (x >> 15) & 1, (x >> 19) & 3)
+                sc = src(z & 0xFFFF, (x >> 16) & 1, (x >> 21) & 3)
                 oracle.oc_compute(fmt, x & 0xFF, (x >> 11) & 7, sa, sb, sc, out,
                                   ctypes.byref(f))
                 flags |= f.value
