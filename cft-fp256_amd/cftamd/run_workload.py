@@ -9,6 +9,13 @@
                                         SUMABS 29, MAXALL 31) over the FMA
                                         operands (binary128 or binary256), 5 modes
     python -m cftamd.run_workload issue    the issue-rate probes (lib/libcftprobe.so)
+    python -m cftamd.run_workload latency  one wavefront alone: its issue interval and
+                                        dependent latency (lib/libcftprobe.so), a
+                                        binary256 FMA / ADD / steered MUL chained in
+                                        registers, the wide runner's chains of 16
+                                        dependent FMA, MUL, ADD and COPYSIGN a trip
+                                        (two lanes an instance), and one Lorenz-63
+                                        trajectory of 20,000 steps
     python -m cftamd.run_workload kernels  the other kernels once each: k_misc
                                         (binary256 MIN over the FMA operands,
                                         --n elements), k_seq (one Lorenz-63
@@ -163,6 +170,72 @@ def issue(args):
               f"({ms.value:.3f} ms a launch)")
 
 
+def _probe():
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
+                        "libcftprobe.so")
+    return ctypes.CDLL(path)
+
+
+def chain_program(op, per_trip=16, fmt=3):
+    """x <- op(x, ...) per_trip times a trip of the segment loop, one value in
+    flight: FMA x*K0 + K1, MUL x*K0, ADD x + K1, COPYSIGN(x, K0); K0 =
+    RN(1.0000001), K1 = RN(1e-30)."""
+    from . import programs as P
+    one = {"fma": P.alu(P.FMA, 1, 1, P.K(0), P.K(1)), "mul": P.alu(P.MUL, 1, 1, P.K(0)),
+           "add": P.alu(P.ADD, 1, 1, 0, P.K(1)), "copysign": P.alu(6, 1, 1, P.K(0))}[op]
+    ins = [P.ldl(1, 0), P.repeat(1)] + [one] * per_trip + [P.endrep(), P.stl(1, 0), P.halt()]
+    return P.image(fmt, ins, consts=[P.dec(fmt, "1.0000001"), P.dec(fmt, "1e-30")],
+                   scratch_in=1, scratch_out=1)
+
+
+def latency(args):
+    from . import programs as P
+    pr = _probe()
+    pr.cftp_lone_issue.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double)]
+    pr.cftp_chain_latency.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    for kind, name in ((0, "v_add_u32"), (1, "v_mad_u64_u32")):
+        for ch in (1, 2, 4, 8):
+            c = ctypes.c_double()
+            if pr.cftp_lone_issue(kind, ch, 20000, ctypes.byref(c)) != 0:
+                raise RuntimeError("the lone-issue probe failed")
+            print(f"lone wave {name}, {ch} chain(s): {c.value:.2f} clocks an instruction")
+    words = np.frombuffer(b"".join(int(v).to_bytes(32, "little") for v in
+                                   (P.dec(3, "1.5"), P.dec(3, "1.0000001"), P.dec(3, "1e-30"))),
+                          np.uint32).copy()
+    out = np.zeros(9, np.uint32)
+    for kind, name in ((0, "FMA"), (1, "ADD (two-term path)"), (2, "MUL (steered to FMA)")):
+        us = ctypes.c_double()
+        if pr.cftp_chain_latency(kind, 20000, words.ctypes.data, out.ctypes.data,
+                                 ctypes.byref(us)) != 0:
+            raise RuntimeError("the chain probe failed")
+        print(f"register chain {name}: {us.value:.3f} us an operation")
+    with Device("hip:0") as dev:
+        for op in ("fma", "mul", "add", "copysign"):
+            prog = Program(dev, chain_program(op))
+            st = odes.widen(np.array([1.5]), 3).copy()
+            prog.run_batch(st.copy(), np.array([100], np.uint32), lanes_per_instance=2)
+            t0 = time.perf_counter()
+            prog.run_batch(st, np.array([10000], np.uint32), lanes_per_instance=2)
+            dt = time.perf_counter() - t0
+            print(f"wide runner chain {op}: {dt / 10000 / 16 * 1e6:.3f} us an operation "
+                  f"(16 a trip, 2 lanes an instance)")
+            prog.free()
+        img, bank, state, steps = odes.lorenz63(1, seed=args.seed, fmt=3)
+        prog = Program(dev, img)
+        for w in (4, 8):
+            s = state.copy()
+            prog.run_batch(s, np.array([200], np.uint32), bank=bank, lanes_per_instance=w)
+            s = state.copy()
+            t0 = time.perf_counter()
+            prog.run_batch(s, np.array([20000], np.uint32), bank=bank, lanes_per_instance=w)
+            dt = time.perf_counter() - t0
+            print(f"lone Lorenz-63, {w} lanes an instance: {dt / 20000 * 1e6:.2f} us a step")
+        prog.free()
+
+
 def kernels(args):
     lib = _lib.lib()
     n = args.n
@@ -204,7 +277,7 @@ def kernels(args):
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m cftamd.run_workload")
     ap.add_argument("workload", choices=["fma", "l63", "l96", "serve", "reduce", "issue",
-                                         "kernels"])
+                                         "latency", "kernels"])
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--fmt", type=int, default=3, choices=[1, 2, 3])
     ap.add_argument("--seg", type=int, default=0)
@@ -213,7 +286,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
     {"fma": fma, "l63": l63, "l96": l96, "serve": serve, "reduce": reduce, "issue": issue,
-     "kernels": kernels}[args.workload](args)
+     "latency": latency, "kernels": kernels}[args.workload](args)
     return 0
 
 

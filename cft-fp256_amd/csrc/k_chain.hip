@@ -104,3 +104,89 @@ extern "C" __attribute__((visibility("default"))) int cftp_chain_latency(int kin
     if (io) (void)hipFree(io);
     return (int)e;
 }
+
+namespace {
+
+// One wavefront, alone on its SIMD: `iters` rounds of 32 instructions of one
+// kind spread over CH independent chains (CH 1: each instruction waits for
+// the one before; CH 8: eight in flight), timed in shader clocks by the wave
+// itself (s_memtime).  Cycles an instruction, for the two limits of a lone
+// wavefront: issue (many chains) and dependent latency (one chain).
+template <int KIND, int CH>
+__global__ __launch_bounds__(64) void k_lone(uint32_t *out, unsigned long long *cyc,
+                                             uint32_t seed, int iters)
+{
+    const uint32_t t = threadIdx.x;
+    const uint32_t a = seed ^ t, b = seed * 2654435761u + t;
+    uint64_t acc[CH];
+    uint32_t v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        acc[k] = (uint64_t)(t + k) << 7;
+        v[k] = t * (k + 3);
+    }
+    const unsigned long long t0 = clock64();
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const int k = q % CH;
+            if constexpr (KIND == 1) {
+                uint64_t cc;
+                asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0"
+                             : "+v"(acc[k]), "=s"(cc)
+                             : "v"(a), "v"(b));
+            } else {
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(v[k]) : "v"(a));
+            }
+        }
+    }
+    const unsigned long long t1 = clock64();
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        s ^= (uint32_t)acc[k] ^ (uint32_t)(acc[k] >> 32) ^ v[k];
+    out[t] = s;
+    if (t == 0)
+        *cyc = t1 - t0;
+}
+
+template <int KIND>
+hipError_t launch_lone(int chains, uint32_t *out, unsigned long long *cyc, int iters)
+{
+    switch (chains) {
+    case 1: hipLaunchKernelGGL((k_lone<KIND, 1>), dim3(1), dim3(64), 0, 0, out, cyc, 7u, iters); break;
+    case 2: hipLaunchKernelGGL((k_lone<KIND, 2>), dim3(1), dim3(64), 0, 0, out, cyc, 7u, iters); break;
+    case 4: hipLaunchKernelGGL((k_lone<KIND, 4>), dim3(1), dim3(64), 0, 0, out, cyc, 7u, iters); break;
+    default: hipLaunchKernelGGL((k_lone<KIND, 8>), dim3(1), dim3(64), 0, 0, out, cyc, 7u, iters); break;
+    }
+    return hipGetLastError();
+}
+
+} // namespace
+
+// Shader clocks an instruction for one lone wavefront: kind 0 v_add_u32, 1
+// v_mad_u64_u32; chains 1, 2, 4 or 8 independent dependence chains.
+// Returns 0 or the HIP error code.
+extern "C" __attribute__((visibility("default"))) int cftp_lone_issue(int kind, int chains,
+                                                                     int iters,
+                                                                     double *clocks_per_insn)
+{
+    if (kind < 0 || kind > 1 || !(chains == 1 || chains == 2 || chains == 4 || chains == 8) ||
+        iters < 1 || !clocks_per_insn)
+        return (int)hipErrorInvalidValue;
+    uint32_t *out = nullptr;
+    unsigned long long *cyc = nullptr;
+    hipError_t e = hipMalloc(&out, 64 * 4);
+    if (e == hipSuccess) e = hipMalloc(&cyc, sizeof *cyc);
+    for (int rep = 0; e == hipSuccess && rep < 2; rep++) {     // the first warms the code
+        e = kind == 0 ? launch_lone<0>(chains, out, cyc, iters)
+                      : launch_lone<1>(chains, out, cyc, iters);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    unsigned long long c = 0;
+    if (e == hipSuccess) e = hipMemcpy(&c, cyc, sizeof c, hipMemcpyDeviceToHost);
+    *clocks_per_insn = (double)c / ((double)iters * 32.0);
+    if (cyc) (void)hipFree(cyc);
+    if (out) (void)hipFree(out);
+    return (int)e;
+}
