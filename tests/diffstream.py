@@ -28,6 +28,13 @@ the checkers).  ``tests/test_diffstream_gpu.py`` runs short streams; the
 command line runs long ones and prints a line a chunk:
 
     python tests/diffstream.py --fmt 3 --op fma --log2n 30 --chunk-log2 24
+    python tests/diffstream.py --path seq --fmt 3 --op fma_negc --log2n 26
+
+With --path seq the same cases go through the sequencer instead: a program
+of one operation (or an ABS of a / a NEG of c ahead of an FMA, which the wide
+scheduler folds into the FMA's operand as a sign modifier), one trajectory of
+one trip a case, through the per-lane runner and the wide runner in turn;
+each lane's result and its own flags (lane_flags) against the oracle.
 """
 
 import argparse
@@ -304,6 +311,121 @@ def stream(fmt, op, total, chunk, seed=1, modes=(0, 1, 2, 3, 4), all_modes=False
     return S
 
 
+# ---- the sequencer on the same cases ----------------------------------------
+
+# program kind -> (the generator's op for its operands, the oracle's op, the
+# operand changes the program makes before it: a fused ABS of a, NEG of c)
+SEQ_KINDS = {"fma": (0, 0, None), "fma_negc": (0, 0, "negc"), "fma_absa": (0, 0, "absa"),
+             "add": (1, 1, None), "sub": (2, 2, None), "mul": (3, 3, None)}
+
+
+def seq_program(kind, rnd, fmt):
+    """slots 0..2 -> r1..r3, one operation (with an ABS of a or a NEG of c
+    ahead of the FMA for the fused kinds), the result to slot 0."""
+    from cftamd import programs as P
+    ins = [P.ldl(1, 0), P.ldl(2, 1), P.ldl(3, 2), P.repeat(1)]
+    if kind == "fma":
+        ins.append(P.alu(P.FMA, 4, 1, 2, 3, rnd=rnd))
+    elif kind == "fma_negc":
+        ins += [P.alu(P.NEG, 5, 3), P.alu(P.FMA, 4, 1, 2, 5, rnd=rnd)]
+    elif kind == "fma_absa":
+        ins += [P.alu(P.ABS, 5, 1), P.alu(P.FMA, 4, 5, 2, 3, rnd=rnd)]
+    elif kind == "add":
+        ins.append(P.alu(P.ADD, 4, 1, 0, 3, rnd=rnd))
+    elif kind == "sub":
+        ins.append(P.alu(P.SUB, 4, 1, 0, 3, rnd=rnd))
+    else:
+        ins.append(P.alu(P.MUL, 4, 1, 2, rnd=rnd))
+    ins += [P.endrep(), P.stl(4, 0), P.halt()]
+    return P.image(fmt, ins, scratch_in=3, scratch_out=3)
+
+
+def seq_check_chunk(dev, O, fmt, kind, rnd, a, b, c, fam, width, threads):
+    """One batched run of the kind's program over the chunk's cases, one
+    trajectory of one trip a case, at `width` lanes an instance (1: the
+    per-lane interpreter; 2 or more: the wide runner): every lane's result
+    and its own flags against the oracle."""
+    from cftamd import Program
+    gop, oop, mod = SEQ_KINDS[kind]
+    n, es = a.shape
+    zero = np.zeros((n, es), np.uint8)
+    xb = b if b is not None else zero
+    xc = c if c is not None else zero
+    state = np.ascontiguousarray(np.stack([a, xb, xc], axis=1)).reshape(-1)
+    lf = np.zeros(n, np.uint8)
+    prog = Program(dev, seq_program(kind, rnd, fmt))
+    try:
+        gfl, status = prog.run_batch(state, np.ones(n, np.uint32), lane_flags=lf,
+                                     lanes_per_instance=width)
+    finally:
+        prog.free()
+    got = state.reshape(n, 3, es)[:, 0, :]
+    oa, oc = a, xc
+    if mod == "absa":
+        oa = a.copy()
+        oa[:, -1] &= 0x7F
+    elif mod == "negc":
+        oc = xc.copy()
+        oc[:, -1] ^= 0x80
+    want, ofl, lanes = oracle_call(O, fmt, oop, rnd, oa, xb, oc, threads)
+    diff = (got != want).any(axis=1)
+    fdiff = (lf & 0x1F) != lanes
+    bad = diff | fdiff
+    return {"cases": n, "differ": int(np.count_nonzero(diff)),
+            "flags_differ": int(np.count_nonzero(fdiff)), "status": int(status),
+            "flag_or_ok": gfl == ofl, "first_bad": [int(i) for i in np.flatnonzero(bad)[:8]],
+            "fam_bad": np.bincount(fam[bad], minlength=NFAM).tolist()}
+
+
+def seq_stream(fmt, kind, total, chunk, seed=1, modes=(0, 1, 2, 3, 4), widths=(1, 2), dev=None,
+               threads=None, log=print):
+    """The sequencer's batched runner on `total` hard cases of the kind's
+    generator stream, chunk j under mode modes[j % len] at width
+    widths[j % len] (so both runners see every mode over enough chunks)."""
+    from cftamd import Device
+    threads = threads or cpu_threads()
+    P_, O = load_probe(), load_oracle()
+    gop = SEQ_KINDS[kind][0]
+    own = dev is None
+    if own:
+        dev = Device("hip:0")
+    S = {"format": fmt, "kind": kind, "seed": seed, "cases": 0, "differ": 0, "flags_differ": 0,
+         "flag_or_bad": 0, "status_bad": 0, "by_width": {}, "per_mode_cases": [0] * 5,
+         "fam_bad": [0] * NFAM, "first_bad": [], "threads": threads}
+    t0 = time.perf_counter()
+    try:
+        nchunks = (total + chunk - 1) // chunk
+        for j in range(nchunks):
+            base = j * chunk
+            n = min(chunk, total - base)
+            a, b, c, fam = generate(P_, fmt, gop, seed, base, n)
+            rnd = modes[j % len(modes)]
+            w = widths[(j // len(modes)) % len(widths)] if nchunks > len(modes) else \
+                widths[j % len(widths)]
+            r = seq_check_chunk(dev, O, fmt, kind, rnd, a, b, c, fam, w, threads)
+            S["cases"] += n
+            S["per_mode_cases"][rnd] += n
+            S["by_width"][w] = S["by_width"].get(w, 0) + n
+            S["differ"] += r["differ"]
+            S["flags_differ"] += r["flags_differ"]
+            S["flag_or_bad"] += 0 if r["flag_or_ok"] else 1
+            S["status_bad"] += 0 if r["status"] == 0 else 1
+            for f in range(NFAM):
+                S["fam_bad"][f] += r["fam_bad"][f]
+            if r["first_bad"] and len(S["first_bad"]) < 8:
+                S["first_bad"] += [(base + i, rnd, w) for i in r["first_bad"]]
+            log(f"diffstream seq fmt {fmt} {kind} chunk {j + 1}/{nchunks} base {base} n {n} "
+                f"rnd {rnd} width {w}: differ {r['differ']}, flags differ {r['flags_differ']}, "
+                f"{time.perf_counter() - t0:.1f} s")
+    finally:
+        if own:
+            dev.close()
+    S["seconds"] = round(time.perf_counter() - t0, 1)
+    S["clean"] = (S["differ"] == 0 and S["flags_differ"] == 0 and S["flag_or_bad"] == 0 and
+                  S["status_bad"] == 0)
+    return S
+
+
 def table(S):
     """A per-family table of one stream's summary (markdown)."""
     lines = [f"| family | cases | differing | " + " | ".join(FLAG_BITS) + " |",
@@ -317,7 +439,11 @@ def table(S):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--fmt", type=int, default=3)
-    ap.add_argument("--op", default="fma", choices=sorted(OPS))
+    ap.add_argument("--op", default="fma", choices=sorted(OPS) + sorted(k for k in SEQ_KINDS
+                                                                      if k not in OPS))
+    ap.add_argument("--path", default="elem", choices=["elem", "seq"],
+                    help="elem: cft_run/cft_div/cft_sqrt; seq: a one-operation program "
+                         "through the batched runners (per-lane and wide)")
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--chunk-log2", type=int, default=22)
     ap.add_argument("--seed", type=int, default=1)
@@ -326,6 +452,15 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--json", default="", help="append the summary to this file")
     args = ap.parse_args()
+    if args.path == "seq":
+        S = seq_stream(args.fmt, args.op, 1 << args.log2n,
+                       1 << min(args.chunk_log2, args.log2n), seed=args.seed,
+                       threads=args.threads or None, log=lambda s: print(s, flush=True))
+        print(json.dumps(S), flush=True)
+        if args.json:
+            with open(args.json, "a") as f:
+                f.write(json.dumps(S) + "\n")
+        sys.exit(0 if S["clean"] else 1)
     S = stream(args.fmt, OPS[args.op], 1 << args.log2n, 1 << min(args.chunk_log2, args.log2n),
                seed=args.seed, all_modes=args.all_modes, threads=args.threads or None,
                log=lambda s: print(s, flush=True))

@@ -45,3 +45,18 @@ def test_stream_short(device, fmt, op):
     assert S["regroup_bits_differ"] == 0
     assert S["clean"]
     assert min(S["fam_cases"]) > 0
+
+
+@pytest.mark.parametrize("fmt", [1, 2, 3])
+@pytest.mark.parametrize("kind", sorted(ds.SEQ_KINDS))
+def test_sequencer_stream_short(device, fmt, kind):
+    """The same hard cases through one-operation programs in the batched
+    runners (per-lane and wide, every mode each), the fused ABS/NEG forms
+    included: each lane's result and its own flags against the oracle."""
+    chunk = 1 << 12
+    S = ds.seq_stream(fmt, kind, 10 * chunk, chunk, seed=77 + fmt, dev=device,
+                      log=lambda s: print(s, flush=True))
+    assert S["differ"] == 0 and S["flags_differ"] == 0, S["first_bad"]
+    assert S["flag_or_bad"] == 0 and S["status_bad"] == 0
+    assert sorted(S["by_width"]) == [1, 2]
+    assert min(S["per_mode_cases"]) > 0
