@@ -196,7 +196,8 @@ def latency(args):
                                    ctypes.POINTER(ctypes.c_double)]
     pr.cftp_chain_latency.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
-    for kind, name in ((0, "v_add_u32"), (1, "v_mad_u64_u32")):
+    for kind, name in ((0, "v_add_u32"), (1, "v_mad_u64_u32"), (2, "ds_bpermute_b32"),
+                       (3, "v_mov_b32 row_shr:1 (DPP)")):
         for ch in (1, 2, 4, 8):
             c = ctypes.c_double()
             if pr.cftp_lone_issue(kind, ch, 20000, ctypes.byref(c)) != 0:
@@ -212,6 +213,15 @@ def latency(args):
                                  ctypes.byref(us)) != 0:
             raise RuntimeError("the chain probe failed")
         print(f"register chain {name}: {us.value:.3f} us an operation")
+    pr.cftp_lp_chain_latency.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_double)]
+    out10 = np.zeros(10, np.uint32)
+    us = ctypes.c_double()
+    if pr.cftp_lp_chain_latency(20000, words.ctypes.data, out10.ctypes.data,
+                                ctypes.byref(us)) != 0:
+        raise RuntimeError("the eight-lane chain probe failed")
+    print(f"register chain FMA over eight lanes (fx_lp.hpp): {us.value:.3f} us an operation "
+          f"({int(out10[9])} handed back)")
     with Device("hip:0") as dev:
         for op in ("fma", "mul", "add", "copysign"):
             prog = Program(dev, chain_program(op))

@@ -135,6 +135,14 @@ __global__ __launch_bounds__(64) void k_lone(uint32_t *out, unsigned long long *
                 asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0"
                              : "+v"(acc[k]), "=s"(cc)
                              : "v"(a), "v"(b));
+            } else if constexpr (KIND == 2) {
+                // a lane's value from the lane one up, through the LDS crossbar
+                v[k] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(((t + 1) & 63) << 2),
+                                                              (int)v[k]);
+            } else if constexpr (KIND == 3) {
+                // the same through the data-parallel move (row_shr:1)
+                v[k] = (uint32_t)__builtin_amdgcn_update_dpp((int)v[k], (int)v[k], 0x111, 0xF,
+                                                             0xF, false);
             } else {
                 asm volatile("v_add_u32 %0, %0, %1" : "+v"(v[k]) : "v"(a));
             }
@@ -165,13 +173,14 @@ hipError_t launch_lone(int chains, uint32_t *out, unsigned long long *cyc, int i
 } // namespace
 
 // Shader clocks an instruction for one lone wavefront: kind 0 v_add_u32, 1
-// v_mad_u64_u32; chains 1, 2, 4 or 8 independent dependence chains.
+// v_mad_u64_u32, 2 ds_bpermute_b32, 3 v_mov_b32 with a DPP row shift; chains
+// 1, 2, 4 or 8 independent dependence chains.
 // Returns 0 or the HIP error code.
 extern "C" __attribute__((visibility("default"))) int cftp_lone_issue(int kind, int chains,
                                                                      int iters,
                                                                      double *clocks_per_insn)
 {
-    if (kind < 0 || kind > 1 || !(chains == 1 || chains == 2 || chains == 4 || chains == 8) ||
+    if (kind < 0 || kind > 3 || !(chains == 1 || chains == 2 || chains == 4 || chains == 8) ||
         iters < 1 || !clocks_per_insn)
         return (int)hipErrorInvalidValue;
     uint32_t *out = nullptr;
@@ -179,8 +188,12 @@ extern "C" __attribute__((visibility("default"))) int cftp_lone_issue(int kind, 
     hipError_t e = hipMalloc(&out, 64 * 4);
     if (e == hipSuccess) e = hipMalloc(&cyc, sizeof *cyc);
     for (int rep = 0; e == hipSuccess && rep < 2; rep++) {     // the first warms the code
-        e = kind == 0 ? launch_lone<0>(chains, out, cyc, iters)
-                      : launch_lone<1>(chains, out, cyc, iters);
+        switch (kind) {
+        case 0: e = launch_lone<0>(chains, out, cyc, iters); break;
+        case 1: e = launch_lone<1>(chains, out, cyc, iters); break;
+        case 2: e = launch_lone<2>(chains, out, cyc, iters); break;
+        default: e = launch_lone<3>(chains, out, cyc, iters); break;
+        }
         if (e == hipSuccess) e = hipDeviceSynchronize();
     }
     unsigned long long c = 0;
