@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
-"""Per-kernel resources of every HIP kernel in libcft.so (and the issue-rate
-probes beside it), from the compiler itself: each kernel source is compiled
+"""Per-kernel resources of every HIP kernel in libcft.so (and the test-support
+kernels of lib/libcftprobe.so beside it), from the compiler itself: each kernel source is compiled
 for gfx950 with the library's own device flags and
 -Rpass-analysis=kernel-resource-usage, and the remarks - VGPRs, AGPRs,
 SGPRs, scratch (spill) bytes a lane, static LDS, the compiler's occupancy
@@ -21,7 +21,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ["k_elem.hip", "k_divsqrt.hip", "k_reduce.hip", "k_seq.hip", "cft_noalloc.hip",
-           "k_probe.hip"]
+           "k_probe.hip", "k_chain.hip", "k_hardcase.hip", "k_lp.hip"]
 FLAGS = ["--offload-arch=gfx950", "-Icsrc", "-O3", "-std=c++17", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "--offload-device-only", "-c", "-o", os.devnull,
          "-Rpass-analysis=kernel-resource-usage"]
@@ -69,10 +69,15 @@ def main():
           "device flags and\n`-Rpass-analysis=kernel-resource-usage` (hipcc, ROCm 7.2). "
           "Command, from the repository root:\n")
     print("    python3 cft-fp256_amd/resource_report.py > profiles/kernel_resources.md\n")
+    spilled = [short(n) for r, n in zip(rows, names)
+               if r.get("ScratchSize [bytes/lane]", "0") not in ("0", "?")]
     print("waves/SIMD is the compiler's bound from registers and static LDS at the "
           "launch bounds; the\nsequencer kernels also take dynamic LDS sized at launch "
           "(cftk_seq_lds, cftk_seq_wide_lds,\ncftk_seq_multi_lds), which bounds them "
-          "lower per program.  No kernel spills (scratch 0).\n")
+          "lower per program.  " +
+          ("No kernel spills (scratch 0)." if not spilled else
+           "Kernels with scratch (spilled registers): " + ", ".join(f"`{k}`" for k in spilled) +
+           ".") + "\n")
     print("| source | kernel | " + " | ".join(h for _k, h in FIELDS) + " |")
     print("|---|---|" + "---|" * len(FIELDS))
     for r, n in zip(rows, names):
