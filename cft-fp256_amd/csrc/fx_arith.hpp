@@ -120,18 +120,19 @@ __device__ __forceinline__ void limb_shl(uint32_t (&x)[N], uint32_t k)
     }
 }
 
-// x >>= 32*k limbs, the dropped limbs ORed into jam; 0 <= k <= N.
+// x >>= 32*k limbs, the dropped limbs ORed into jam; 0 <= k < N (a select
+// level per power of two below N).
 template <int N>
 __device__ __forceinline__ void limb_shr_jam(uint32_t (&x)[N], uint32_t k, uint32_t &jam)
 {
 #pragma unroll
     for (int t = 16; t >= 1; t >>= 1) {
-        if (t > N)
+        if (t >= N)
             continue;
         const bool s = (k & (uint32_t)t) != 0;
         uint32_t dropped = 0;
 #pragma unroll
-        for (int i = 0; i < (t < N ? t : N); i++)
+        for (int i = 0; i < t; i++)
             dropped |= x[i];
         jam |= s ? dropped : 0u;
 #pragma unroll
@@ -141,12 +142,13 @@ __device__ __forceinline__ void limb_shr_jam(uint32_t (&x)[N], uint32_t k, uint3
 }
 
 // x >>= r bits (any r >= 0), every bit shifted out ORed into jam (nonzero
-// iff something was dropped).
+// iff something was dropped).  A shift past 32N-1 is taken as 32N-1: every
+// caller's top bit (bit 32N-1) is clear, so that empties x all the same.
 template <int N>
 __device__ __forceinline__ void shr_jam(uint32_t (&x)[N], uint32_t r, uint32_t &jam)
 {
-    if (r > 32u * N)
-        r = 32u * N;
+    if (r > 32u * N - 1u)
+        r = 32u * N - 1u;
     const uint32_t k = r >> 5, b = r & 31u;
     limb_shr_jam<N>(x, k, jam);
     jam |= x[0] & ((1u << b) - 1u);
@@ -266,22 +268,25 @@ __device__ __forceinline__ void enc_maxnorm(uint32_t (&o)[NL], int sign)
     o[NL - 1] = ((uint32_t)sign << 31) | ((Fm::EMASK - 1u) << Fm::TOPF) | Fm::FMASK;
 }
 
-// host/src/softfloat.c:120-134
+// host/src/softfloat.c:120-134, as a truth table a mode over the index
+// (sign, g, s, lsb): RNE g&(s|lsb) 0xE0E0, RTZ 0, RDN sign&(g|s) 0xFC00,
+// RUP !sign&(g|s) 0x00FC, RMM g 0xF0F0.  A lookup, not a switch: where the
+// mode differs by lane (the sequencer's records) a switch is a chain of
+// divergent branches.
 __device__ __forceinline__ bool round_incr(int rnd, int sign, bool g, bool s, bool lsb)
 {
-    switch (rnd) {
-    case RNE: return g && (s || lsb);
-    case RTZ: return false;
-    case RDN: return sign && (g || s);
-    case RUP: return !sign && (g || s);
-    default:  return g;
-    }
+    const uint32_t idx = ((uint32_t)sign << 3) | ((uint32_t)g << 2) | ((uint32_t)s << 1) |
+                         (uint32_t)lsb;
+    const uint32_t t = rnd == RMM ? 0xF0F0u
+                                  : (uint32_t)(0x00FCFC000000E0E0ull >> (16 * (rnd & 3)));
+    return (t >> idx) & 1u;
 }
 
-// host/src/softfloat.c:141-149
+// host/src/softfloat.c:141-149: bit 2*rnd + sign of 0x363 (RNE and RMM
+// always, RDN for a negative result, RUP for a positive one, RTZ never).
 __device__ __forceinline__ bool overflow_to_inf(int rnd, int sign)
 {
-    return rnd == RTZ ? false : rnd == RDN ? sign != 0 : rnd == RUP ? sign == 0 : true;
+    return (0x363u >> (2 * rnd + sign)) & 1u;
 }
 
 // k (NL limbs, < 2^P) += 1.  Returns true when it carried to exactly 2^P.
@@ -311,14 +316,19 @@ __device__ __forceinline__ void round_pack_norm(int sign, uint32_t (&K)[NL], boo
     using Fm = F<NL>;
     if (e_norm >= Fm::EMIN) {
         const bool inx = g || s;
-        if (round_incr(rnd, sign, g, s, K[0] & 1u)) {
-            if (inc_sig<NL>(K)) {               // 2^P: renormalise, exact
+        {
+            // K += the increment, one carry chain; a carry to 2^P leaves
+            // every lower limb zero, so renormalising is the top limb
+            uint32_t c = round_incr(rnd, sign, g, s, K[0] & 1u) ? 1u : 0u;
 #pragma unroll
-                for (int i = 0; i < NL - 1; i++)
-                    K[i] = 0;
-                K[NL - 1] = 1u << Fm::TOPF;
-                e_norm++;
+            for (int i = 0; i < NL; i++) {
+                uint32_t co;
+                K[i] = addc(K[i], 0u, c, &co);
+                c = co;
             }
+            const bool up = (K[NL - 1] >> Fm::TOPF) & 2u;
+            K[NL - 1] = up ? (1u << Fm::TOPF) : K[NL - 1];
+            e_norm += up ? 1 : 0;
         }
         if (e_norm > Fm::EMAX) {
             if (overflow_to_inf(rnd, sign))
@@ -545,13 +555,16 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
         }
         return;
     }
-    if (ua.sub || ub.sub || uc.sub) {            // rare: normalise subnormals
+    // MUL rounds from the product's top bit, which needs normal factors.  FMA,
+    // ADD and SUB leave a subnormal significand raw: each term is placed by
+    // its lsb exponent, so a subnormal one merely has its leading bit below
+    // T, and the sum then takes the general normalisation below.
+    if (op == OP_MUL && (ua.sub || ub.sub)) {
         normalise<NL>(ua);
         normalise<NL>(ub);
-        normalise<NL>(uc);
     }
 
-    // the exact product, leading bit at 2P-2 or 2P-1
+    // the exact product, leading bit at 2P-2 or 2P-1 for normal factors
     uint32_t Pm[WN];
     if (op == OP_ADD || op == OP_SUB) {
 #pragma unroll
@@ -579,7 +592,8 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
         return;
     }
 
-    // Pw = P with its leading bit at T: a shift of T-TB or T-TB+1 bits
+    // Pw = P with its leading bit at T (bit 2P-2 at T when the top bit is
+    // clear, normal or not): a shift of T-TB or T-TB+1 bits
     uint32_t Pw[WN], Cw[WN];
     {
         constexpr int SH = T - TB;               // when ptop
@@ -598,7 +612,8 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
         }
     }
     const int VEp = ep + (ptop ? 2 * P - 1 : 2 * P - 2);
-    // Cw = C with its leading bit at T (a constant shift)
+    // Cw = C with bit P-1 at T (a constant shift): its leading bit, or above
+    // it for a subnormal C
     {
         constexpr int SH = T - (P - 1), Q = SH / 32, R = SH % 32;
 #pragma unroll
@@ -624,7 +639,7 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     {
         uint32_t jam = 0;
         const int ad = d < 0 ? -d : d;
-        shr_jam<WN>(small, (uint32_t)(ad > 32 * WN ? 32 * WN : ad), jam);
+        shr_jam<WN>(small, (uint32_t)(ad > 32 * WN - 1 ? 32 * WN - 1 : ad), jam);
         small[0] |= jam ? 1u : 0u;
     }
     const int VEb = cbig ? VEc : VEp;            // the exponent of bit T
@@ -642,9 +657,10 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     const bool tm = (S[(T - 1) / 32] >> ((T - 1) % 32)) & 1u;
     const bool neg = (S[WN - 1] >> 31) & 1u;     // only when d == 0, subtracting
     int sign = bsign;
-    int M;
     if (neg || (!t1 && !t0 && !tm)) {
-        // rare: equal leading exponents cancelled (or the smaller term won)
+        // rare: equal leading exponents cancelled, the smaller term won, or
+        // the larger term was subnormal (a raw significand, or a product
+        // with a subnormal factor) and sits below T
         if (neg) {
             neg_limbs<WN>(S);
             sign ^= 1;
@@ -655,11 +671,16 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
             return;
         }
         shl<WN>(S, (uint32_t)(T + 1 - (L - 1)));  // leading bit to T+1
-        M = T + 1;
-        round_fixed<NL, WN, T>(sign, S, M, VEb + (L - 1) - T, rnd, out, flags);
+        round_fixed<NL, WN, T>(sign, S, T + 1, VEb + (L - 1) - T, rnd, out, flags);
         return;
     }
-    M = t1 ? T + 1 : (t0 ? T : T - 1);
+    // Whenever the jam bit was set, the sum's rounding position lies at bit 2
+    // or above: the leading bit is at T-1 or higher with both terms normal;
+    // when the smaller term was shifted out past bit 0 under a subnormal
+    // larger one, the result is below 2^emin and rounds at C's bit 0, at
+    // T-P+1; and a product with a subnormal factor that is the larger term
+    // only drops bits of a C at least 2^(T-P+1) times smaller.
+    const int M = t1 ? T + 1 : (t0 ? T : T - 1);
     round_fixed<NL, WN, T>(sign, S, M, VEb + (M - T), rnd, out, flags);
 }
 
