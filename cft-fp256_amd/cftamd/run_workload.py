@@ -1,6 +1,8 @@
 """Run one of the benchmark's workloads once, for profilers and quick checks.
 
     python -m cftamd.run_workload fma   [--n N] [--seed S]   binary256 FMA, 5 modes
+    python -m cftamd.run_workload divsqrt [--n N] [--seed S] binary256 division, then
+                                        square root of |a|, 5 modes each
     python -m cftamd.run_workload l63   [--seed S]           Lorenz-63 ensemble, one launch
     python -m cftamd.run_workload l96   [--fmt 1|3] [--seed S]  Lorenz-96, one launch
     python -m cftamd.run_workload serve [--seconds T] [--seed S]  the serving stream
@@ -70,6 +72,37 @@ def fma(args):
         dt = time.perf_counter() - t0
         k = len(modes) * n
         print(f"fma: {k} elements in {dt:.4f} s ({k / dt / 1e9:.3f} Gelem/s), flags {fl.value}")
+        for b in bufs:
+            b.free()
+
+
+def divsqrt(args):
+    """binary256 division then square root (of |a|), the five rounding modes
+    each, over the elementwise configuration's operands."""
+    lib = _lib.lib()
+    n = args.n
+    with Device("hip:0") as dev:
+        bufs = [dev.alloc(n * 32) for _ in range(4)]
+        workload.elementwise(2, n, seed=args.seed, threads=16,
+                             out=(bufs[0].host, bufs[1].host, bufs[2].host))
+        bufs[2].host[:] = bufs[0].host                   # sqrt reads |a|
+        bufs[2].host.reshape(n, 32)[:, 31] &= 0x7F
+        for b in bufs[:3]:
+            b.to_device()
+        fl = ctypes.c_uint32()
+        for name in ("div", "sqrt"):
+            t0 = time.perf_counter()
+            for rnd in range(5):
+                if name == "div":
+                    st = lib.cft_div(dev.handle, 3, rnd, bufs[0].data_ptr, bufs[1].data_ptr,
+                                     bufs[3].data_ptr, n, ctypes.byref(fl), None)
+                else:
+                    st = lib.cft_sqrt(dev.handle, 3, rnd, bufs[2].data_ptr, bufs[3].data_ptr, n,
+                                      ctypes.byref(fl), None)
+                _lib.check(st, "cft_" + name)
+            dt = time.perf_counter() - t0
+            print(f"{name}: {5 * n} elements in {dt:.4f} s ({5 * n / dt / 1e9:.3f} Gelem/s), "
+                  f"flags {fl.value}")
         for b in bufs:
             b.free()
 
@@ -286,7 +319,8 @@ def kernels(args):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m cftamd.run_workload")
-    ap.add_argument("workload", choices=["fma", "l63", "l96", "serve", "reduce", "issue",
+    ap.add_argument("workload", choices=["fma", "divsqrt", "l63", "l96", "serve", "reduce",
+                                         "issue",
                                          "latency", "kernels"])
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--fmt", type=int, default=3, choices=[1, 2, 3])
@@ -295,7 +329,8 @@ def main(argv=None):
     ap.add_argument("--seconds", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
-    {"fma": fma, "l63": l63, "l96": l96, "serve": serve, "reduce": reduce, "issue": issue,
+    {"fma": fma, "divsqrt": divsqrt, "l63": l63, "l96": l96, "serve": serve, "reduce": reduce,
+     "issue": issue,
      "latency": latency, "kernels": kernels}[args.workload](args)
     return 0
 

@@ -45,9 +45,9 @@ __device__ __forceinline__ void long_div(uint32_t (&U)[2 * NL + 1], const uint32
         // by Knuth's truncated-divisor bound, also for the cap): the add-back
         // below almost never runs.  The estimate is never negative, so one
         // minimum is the whole clamp.
-        const double num = (double)U[j + NL] * 18446744073709551616.0 +
-                           (double)U[j + NL - 1] * 4294967296.0 +
-                           (double)(j + NL - 2 >= 0 ? U[(j + NL - 2) < 0 ? 0 : j + NL - 2] : 0u);
+        const double num =
+            __builtin_fma((double)U[j + NL - 1], 0x1p32, (double)U[j + NL] * 0x1p64) +
+            (double)(j + NL - 2 >= 0 ? U[(j + NL - 2) < 0 ? 0 : j + NL - 2] : 0u);
         uint32_t qhat = (uint32_t)__builtin_fmin(num * rinv + 0x1p-17, 4294967295.0);
         // U[j .. j+NL] -= qhat * D: the NL partial products independently,
         // their limbs assembled by one carry chain and subtracted by one
@@ -97,12 +97,65 @@ __device__ __forceinline__ void long_div(uint32_t (&U)[2 * NL + 1], const uint32
     rem_nz = o != 0;
 }
 
+// An operand the finite cores take as it is: finite, nonzero and normal (its
+// exponent field neither 0 nor all ones).  Anything else goes through the
+// full div_op / sqrt_op, which unpack, settle the specials and normalise.
+template <int NL>
+__device__ __forceinline__ bool plain_operand(const uint32_t (&x)[NL])
+{
+    using Fm = F<NL>;
+    const uint32_t ef = (x[NL - 1] >> Fm::TOPF) & Fm::EMASK;
+    return ef != 0 && ef != Fm::EMASK;
+}
+
+// a / b for finite nonzero operands, unpacked and normalised.
+template <int NL>
+__device__ __forceinline__ void div_finite(const Unp<NL> &ua, const Unp<NL> &ub, int rnd,
+                                           uint32_t (&out)[NL], uint32_t &flags)
+{
+    using Fm = F<NL>;
+    constexpr int P = Fm::P;
+    const int sq = ua.sign ^ ub.sign;
+    // Q = floor(ma * 2^k / mb) with exactly P+3 bits: k = P+2 when ma >= mb
+    bool ge = true;
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        ge = ua.m[i] != ub.m[i] ? ua.m[i] > ub.m[i] : ge;
+    const int k = ge ? P + 2 : P + 3;
+    constexpr int SD = 32 * NL - P;              // divisor normalisation
+    uint32_t D[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        D[i] = ub.m[i];
+    shl<NL>(D, SD);
+    // U = ma << (k + SD) = (ma << (k + SD - 32NL)) in the top NL limbs
+    // k + SD - 32NL is 3, or 2 when ma >= mb: one funnel shift a limb (ma's
+    // top bit, P-1, stays inside the top limb either way)
+    static_assert(P + 3 <= 32 * NL, "the shifted dividend must fit its limbs");
+    uint32_t U[2 * NL + 1];
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        U[i] = 0;
+        U[NL + i] = alignbit(ua.m[i], i ? ua.m[i - 1] : 0u, ge ? 30u : 29u);
+    }
+    U[2 * NL] = 0;
+    uint32_t Qd[NL];
+    bool rem_nz;
+    long_div<NL>(U, D, Qd, rem_nz);
+    // Q has P+3 bits: K = Q >> 3, guard bit 2, sticky bits 1..0 and the rest
+    uint32_t K[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        K[i] = alignbit(i + 1 < NL ? Qd[i + 1] : 0u, Qd[i], 3);
+    const bool g = (Qd[0] >> 2) & 1u;
+    const bool s = (Qd[0] & 3u) != 0 || rem_nz;
+    round_pack_norm<NL>(sq, K, g, s, ua.e - ub.e - k + P + 2, rnd, out, flags);
+}
+
 template <int NL>
 __device__ __forceinline__ void div_op(const uint32_t (&xa)[NL], const uint32_t (&xb)[NL], int rnd,
                                        uint32_t (&out)[NL], uint32_t &flags)
 {
-    using Fm = F<NL>;
-    constexpr int P = Fm::P;
     Unp<NL> ua, ub;
     unpack<NL>(xa, ua);
     unpack<NL>(xb, ub);
@@ -124,48 +177,7 @@ __device__ __forceinline__ void div_op(const uint32_t (&xa)[NL], const uint32_t 
         return;
     }
     if (ua.zero) { enc_zero<NL>(out, sq); return; }
-
-    // Q = floor(ma * 2^k / mb) with exactly P+3 bits: k = P+2 when ma >= mb
-    bool ge = true;
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        ge = ua.m[i] != ub.m[i] ? ua.m[i] > ub.m[i] : ge;
-    const int k = ge ? P + 2 : P + 3;
-    constexpr int SD = 32 * NL - P;              // divisor normalisation
-    uint32_t D[NL];
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        D[i] = ub.m[i];
-    shl<NL>(D, SD);
-    // U = ma << (k + SD) = (ma << (k + SD - 32NL)) in the top NL limbs
-    uint32_t U[2 * NL + 1];
-    uint32_t hi[NL];
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        hi[i] = ua.m[i];
-    // k + SD - 32NL is 2 or 3: the constant 3 (folded), then one bit back
-    // when ma >= mb (ma << 3 has its low bits clear)
-    shl<NL>(hi, 3u);
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        hi[i] = alignbit(i + 1 < NL ? hi[i + 1] : 0u, hi[i], ge ? 1u : 0u);
-#pragma unroll
-    for (int i = 0; i < NL; i++) {
-        U[i] = 0;
-        U[NL + i] = hi[i];
-    }
-    U[2 * NL] = 0;
-    uint32_t Qd[NL];
-    bool rem_nz;
-    long_div<NL>(U, D, Qd, rem_nz);
-    // Q has P+3 bits: K = Q >> 3, guard bit 2, sticky bits 1..0 and the rest
-    uint32_t K[NL];
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        K[i] = alignbit(i + 1 < NL ? Qd[i + 1] : 0u, Qd[i], 3);
-    const bool g = (Qd[0] >> 2) & 1u;
-    const bool s = (Qd[0] & 3u) != 0 || rem_nz;
-    round_pack_norm<NL>(sq, K, g, s, ua.e - ub.e - k + P + 2, rnd, out, flags);
+    div_finite<NL>(ua, ub, rnd, out, flags);
 }
 
 // r = floor(sqrt(M)), M 2*NL limbs with its top bit at 64NL-1 or 64NL-2;
@@ -182,19 +194,28 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
 #pragma unroll
     for (int i = 0; i < NL; i++)
         r[i] = 0;
-    // first digit: isqrt of the top 64 bits
+    // first digit: isqrt of the top 64 bits.  g as a double (one rounding,
+    // 2^-53 relative), its root (one more): within 2^-21 of sqrt(g) < 2^32,
+    // so the truncated root is floor(sqrt(g)) or one off either way, and one
+    // exact step each way settles it.
     {
         const uint64_t g = ((uint64_t)M[2 * NL - 1] << 32) | M[2 * NL - 2];
-        uint64_t d = (uint64_t)sqrt((double)g);
-        if (d > 0xFFFFFFFFull) d = 0xFFFFFFFFull;
-        while (d * d > g) d--;
-        while (d < 0xFFFFFFFFull && (d + 1) * (d + 1) <= g) d++;
-        const uint64_t rem = g - d * d;
+        const double gd = __builtin_fma((double)M[2 * NL - 1], 0x1p32, (double)M[2 * NL - 2]);
+        uint32_t d = (uint32_t)__builtin_fmin(sqrt(gd), 4294967295.0);
+        uint64_t sq = (uint64_t)d * d;
+        const bool over = sq > g;
+        const uint64_t up = sq + 2ull * d + 1ull;        // (d+1)^2
+        const bool under = !over && d != 0xFFFFFFFFu && up <= g;
+        sq = over ? sq - 2ull * d + 1ull : under ? up : sq;
+        d = over ? d - 1u : under ? d + 1u : d;
+        const uint64_t rem = g - sq;
         R[0] = (uint32_t)rem;
         R[1] = (uint32_t)(rem >> 32);
-        r[NL - 1] = (uint32_t)d;                 // r kept left-aligned: digits fill down
+        r[NL - 1] = d;                           // r kept left-aligned: digits fill down
     }
-    double rinv = 0.0;                           // 1 / (V B), set at the third digit
+    // 1 / (V B) from the third digit on, scaled by 2^(32(k-1)) like Rs below:
+    // set at the third digit and, so scaled, the same for every later one
+    double rinv = 0.0;
 #pragma unroll
     for (int k = 1; k < NL; k++) {
         // r so far: digits r[NL-k .. NL-1] (k limbs); R < 2^33 * 2^(32(k-1))
@@ -215,18 +236,21 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
             V[i] = alignbit(r[NL - k + i], lo, 31);
         }
         V[k] = r[NL - 1] >> 31;
-        // estimate d = R / (V * B), from the top limbs as doubles
-        double Rd = 0.0, Vd = 0.0;
-#pragma unroll
-        for (int i = 0; i < NL + 2; i++)
-            if (i <= k + 2 && i >= k - 1)
-                Rd += ldexp((double)R[i], 32 * i);
-        if (k <= 2) {
-#pragma unroll
-            for (int i = 0; i <= NL; i++)
-                if (i <= k)
-                    Vd += ldexp((double)V[i], 32 * (i + 1));
-        }
+        // estimate d = R / (V * B), from the top limbs as doubles.  Rs is R's
+        // limbs k-1 .. k+2 summed low to high (each step one rounding) at
+        // the scale 2^(-32(k-1)); Vd (the second and third digits) V B at
+        // the scale 2^(-32(k-1)).  Powers of two commute with rounding, so
+        // these are the unscaled sums' bits, scaled.
+        double Rs = (double)R[k - 1];
+        Rs = __builtin_fma((double)R[k], 0x1p32, Rs);
+        Rs = __builtin_fma((double)R[k + 1], 0x1p64, Rs);
+        Rs = __builtin_fma((double)R[k + 2], 0x1p96, Rs);
+        double Vd = 0.0;
+        if (k == 1)
+            Vd = __builtin_fma((double)V[1], 0x1p64, (double)V[0] * 0x1p32);
+        else if (k == 2)
+            Vd = __builtin_fma((double)V[2], 0x1p64, __builtin_fma((double)V[1], 0x1p32,
+                                                                 (double)V[0]));
         // At or above the digit: R's dropped limbs move R / (V B) by under
         // 2^-63 (V B >= 2^64), V's only raise the estimate, and the sums and
         // the quotient (below 2^33) carry 6.5 * 2^-53 relative, 2^-17.3
@@ -236,24 +260,26 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         // is under 2^-32 (V B >= 2^96), but for the second (V B >= 2^64) it
         // stepped about 30% of lanes, so nearly every wavefront.  There the
         // digit is floor(x) for the root x of (V B + x) x = R, taken as
-        // x = 2R / (V B + sqrt(V^2 B^2 + 4R)), which has no cancellation: Rd
+        // x = 2R / (V B + sqrt(V^2 B^2 + 4R)), which has no cancellation: Rs
         // is R with three roundings (all of R's limbs), Vd = V B exactly,
         // and the square, the sum, the root, the sum and the quotient bring
         // the whole to at most 8 * 2^-53 relative, under 2^-18 absolute for
         // x < 2^32.  + 2^-15 again never undershoots.
         // From the fourth digit on, V B is the third digit's Vd times
-        // 2^(32(k-2)), its reciprocal the third's times 2^(-32(k-2)): that drops V's
-        // limbs below (each digit's 2d, under 2^-63 of V B), which only
-        // raises the estimate, by under 2^-30; Rd * rinv carries at most
-        // 8 * 2^-53 relative of rounding (Rd three, Vd two, the reciprocal,
+        // 2^(32(k-2)), its reciprocal the third's times 2^(-32(k-2)) (at the
+        // running scale, the third's itself): that drops V's limbs below (each
+        // digit's 2d, under 2^-63 of V B), which only raises the estimate, by
+        // under 2^-30; Rs * rinv carries at most
+        // 8 * 2^-53 relative of rounding (Rs three, Vd two, the reciprocal,
         // the product, the bias sum), 2^-17 absolute, and + 2^-15 never
         // undershoots.
         double dd;
         if (k == 1) {
-            dd = 2.0 * Rd / (Vd + sqrt(Vd * Vd + 4.0 * Rd)) + 0x1p-15;
+            dd = 2.0 * Rs / (Vd + sqrt(Vd * Vd + 4.0 * Rs)) + 0x1p-15;
         } else {
-            rinv = k == 2 ? 1.0 / Vd : rinv * 0x1p-32;
-            dd = Rd * rinv + 0x1p-15;
+            if (k == 2)
+                rinv = 1.0 / Vd;
+            dd = Rs * rinv + 0x1p-15;
         }
         uint32_t d = (uint32_t)__builtin_fmin(dd, 4294967295.0);   // dd > 0
         // R -= T, T = (V*B + d) * d: the limbs of V*B + d are [d, V0 .. Vk];
@@ -318,12 +344,47 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
     rem_nz = o != 0;
 }
 
+// sqrt(a) for a positive normal operand, unpacked and normalised.
+template <int NL>
+__device__ __forceinline__ void sqrt_finite(const Unp<NL> &ua, int rnd, uint32_t (&out)[NL],
+                                            uint32_t &flags)
+{
+    using Fm = F<NL>;
+    constexpr int P = Fm::P;
+    // M = m << t, top bit at 64NL-1 or 64NL-2, (e - t) even: t is the
+    // constant 64NL - P, or one less; both shifts move m by the same whole
+    // limbs Q and then R or R-1 bits, one funnel shift a limb
+    constexpr int SH = 64 * NL - P, Q = SH / 32, R = SH % 32;
+    static_assert(R >= 2, "both shifts within one limb");
+    const uint32_t back = (uint32_t)((ua.e - SH) & 1);
+    const int t = SH - (int)back;
+    const uint32_t rb = (uint32_t)R - back;
+    uint32_t M[2 * NL];
+#pragma unroll
+    for (int i = 0; i < 2 * NL; i++) {
+        const int j = i - Q;
+        const uint32_t hi = (j >= 0 && j < NL) ? ua.m[j] : 0u;
+        const uint32_t lo = (j - 1 >= 0 && j - 1 < NL) ? ua.m[j - 1] : 0u;
+        M[i] = (j < 0 || j > NL) ? 0u : alignbit(hi, lo, 32u - rb);
+    }
+    uint32_t r[NL];
+    bool rem_nz;
+    isqrt_limbs<NL>(M, r, rem_nz);
+    // r in [2^(32NL-1), 2^(32NL)): K = top P bits, then guard and sticky
+    constexpr int DROP = 32 * NL - P;            // = EXPW
+    uint32_t K[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++)
+        K[i] = alignbit(i + 1 < NL ? r[i + 1] : 0u, r[i], DROP);
+    const bool g = (r[0] >> (DROP - 1)) & 1u;
+    const bool s = (r[0] & ((1u << (DROP - 1)) - 1u)) != 0 || rem_nz;
+    round_pack_norm<NL>(0, K, g, s, 32 * NL - 1 + (ua.e - t) / 2, rnd, out, flags);
+}
+
 template <int NL>
 __device__ __forceinline__ void sqrt_op(const uint32_t (&xa)[NL], int rnd, uint32_t (&out)[NL],
                                         uint32_t &flags)
 {
-    using Fm = F<NL>;
-    constexpr int P = Fm::P;
     Unp<NL> ua;
     unpack<NL>(xa, ua);
     if (ua.nan) {
@@ -342,32 +403,7 @@ __device__ __forceinline__ void sqrt_op(const uint32_t (&xa)[NL], int rnd, uint3
         flags |= FL_INV;
         return;
     }
-    // M = m << t, top bit at 64NL-1 or 64NL-2, (e - t) even: a shift by the
-    // constant 64NL - P (folded at compile time), then one bit back when t is
-    // one less (m << 64NL-P has its low bits clear, so nothing is lost)
-    int t = 64 * NL - P;                         // top bit at 64NL - 1
-    const uint32_t back = (uint32_t)((ua.e - t) & 1);
-    t -= (int)back;
-    uint32_t M[2 * NL];
-#pragma unroll
-    for (int i = 0; i < 2 * NL; i++)
-        M[i] = i < NL ? ua.m[i] : 0u;
-    shl<2 * NL>(M, (uint32_t)(64 * NL - P));
-#pragma unroll
-    for (int i = 0; i < 2 * NL; i++)
-        M[i] = alignbit(i + 1 < 2 * NL ? M[i + 1] : 0u, M[i], back);
-    uint32_t r[NL];
-    bool rem_nz;
-    isqrt_limbs<NL>(M, r, rem_nz);
-    // r in [2^(32NL-1), 2^(32NL)): K = top P bits, then guard and sticky
-    constexpr int DROP = 32 * NL - P;            // = EXPW
-    uint32_t K[NL];
-#pragma unroll
-    for (int i = 0; i < NL; i++)
-        K[i] = alignbit(i + 1 < NL ? r[i + 1] : 0u, r[i], DROP);
-    const bool g = (r[0] >> (DROP - 1)) & 1u;
-    const bool s = (r[0] & ((1u << (DROP - 1)) - 1u)) != 0 || rem_nz;
-    round_pack_norm<NL>(0, K, g, s, 32 * NL - 1 + (ua.e - t) / 2, rnd, out, flags);
+    sqrt_finite<NL>(ua, rnd, out, flags);
 }
 
 } // namespace fx
