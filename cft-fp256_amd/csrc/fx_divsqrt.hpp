@@ -245,40 +245,37 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         Rs = __builtin_fma((double)R[k], 0x1p32, Rs);
         Rs = __builtin_fma((double)R[k + 1], 0x1p64, Rs);
         Rs = __builtin_fma((double)R[k + 2], 0x1p96, Rs);
-        double Vd = 0.0;
-        if (k == 1)
-            Vd = __builtin_fma((double)V[1], 0x1p64, (double)V[0] * 0x1p32);
-        else if (k == 2)
-            Vd = __builtin_fma((double)V[2], 0x1p64, __builtin_fma((double)V[1], 0x1p32,
-                                                                 (double)V[0]));
         // At or above the digit: R's dropped limbs move R / (V B) by under
-        // 2^-63 (V B >= 2^64), V's only raise the estimate, and the sums and
-        // the quotient (below 2^33) carry 6.5 * 2^-53 relative, 2^-17.3
-        // absolute, of rounding; the digit itself is at most R / (V B).  So
-        // + 2^-15 never undershoots, and the add-back below runs for the
-        // d^2 / (V B) < 1 the estimate ignores: from the third digit on that
-        // is under 2^-32 (V B >= 2^96), but for the second (V B >= 2^64) it
-        // stepped about 30% of lanes, so nearly every wavefront.  There the
-        // digit is floor(x) for the root x of (V B + x) x = R, taken as
-        // x = 2R / (V B + sqrt(V^2 B^2 + 4R)), which has no cancellation: Rs
-        // is R with three roundings (all of R's limbs), Vd = V B exactly,
-        // and the square, the sum, the root, the sum and the quotient bring
-        // the whole to at most 8 * 2^-53 relative, under 2^-18 absolute for
-        // x < 2^32.  + 2^-15 again never undershoots.
-        // From the fourth digit on, V B is the third digit's Vd times
+        // 2^-63 (V B >= 2^64), V's only raise the estimate, and the digit is
+        // floor(x) for the root x of (V B + x) x = R.
+        // Second digit (V B = 2 r B, 33 significant bits, exact as a double,
+        // rinv its correctly rounded reciprocal): with q = R / (V B) and
+        // u = x / (V B) <= 2^-32, x = q / (1 + u) and q - q^2 / (V B) =
+        // q (1 - u - u^2), so the one Newton step q - q * (q * rinv) lies at
+        // most 2 q u^2 <= 2^-30 below x; q carries five roundings (Rs three,
+        // rinv, the product), 2^-17.7 absolute at q < 2^33, the step two more
+        // (2^-21): + 2^-15 never undershoots, and overshoots by less than 1,
+        // so the add-back below runs only when x lies within 2^-15 of an
+        // integer from below.
+        // Third digit: V B there is (the second's V B + 2 d1) at the running
+        // scale, so its reciprocal is rinv / (1 + e), e = 2 d1 rinv <
+        // 2^-31, taken as rinv (1 - e): low by under e^2 = 2^-62 relative
+        // (2^-29 absolute on the estimate) with two roundings.
+        // From the fourth digit on, V B is the third digit's times
         // 2^(32(k-2)), its reciprocal the third's times 2^(-32(k-2)) (at the
         // running scale, the third's itself): that drops V's limbs below (each
         // digit's 2d, under 2^-63 of V B), which only raises the estimate, by
-        // under 2^-30; Rs * rinv carries at most
-        // 8 * 2^-53 relative of rounding (Rs three, Vd two, the reciprocal,
-        // the product, the bias sum), 2^-17 absolute, and + 2^-15 never
-        // undershoots.
+        // under 2^-30.  Rs * rinv carries at most 8 * 2^-53 relative of
+        // rounding, 2^-17 absolute, which with the 2^-29 is below the + 2^-15:
+        // it never undershoots.
         double dd;
         if (k == 1) {
-            dd = 2.0 * Rs / (Vd + sqrt(Vd * Vd + 4.0 * Rs)) + 0x1p-15;
+            rinv = 1.0 / __builtin_fma((double)V[1], 0x1p64, (double)V[0] * 0x1p32);
+            const double qd = Rs * rinv;
+            dd = __builtin_fma(-qd, qd * rinv, qd) + 0x1p-15;
         } else {
             if (k == 2)
-                rinv = 1.0 / Vd;
+                rinv = __builtin_fma(-rinv, 2.0 * (double)r[NL - 2] * rinv, rinv);
             dd = Rs * rinv + 0x1p-15;
         }
         uint32_t d = (uint32_t)__builtin_fmin(dd, 4294967295.0);   // dd > 0

@@ -134,3 +134,108 @@ extern "C" int emu_hardcases(int fmt, int op, uint64_t seed, uint64_t base, size
     }
     return 0;
 }
+
+// The square root's digit recurrence (isqrt_limbs, binary256 width) held to
+// what fixes it: r^2 <= M < (r+1)^2 and rem_nz == (r^2 != M), exactly, on n
+// radicands from a seeded generator - random ones (top bit at 511 or 510)
+// and, at three in four, squares s^2 of random 256-bit roots plus -1, 0,
+// +1, 2s or 2s-1, some with runs of zero or all-one limbs, where the digit
+// estimates sit on integer boundaries.  Returns the number that fail.
+static int cmp16(const uint32_t *a, const uint32_t *b)
+{
+    for (int i = 15; i >= 0; i--)
+        if (a[i] != b[i])
+            return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+
+static uint64_t sm64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+extern "C" size_t emu_isqrt_check(uint64_t seed, size_t n)
+{
+    constexpr int NL = 8;
+    size_t bad = 0;
+    uint64_t st = seed;
+    for (size_t k = 0; k < n; k++) {
+        uint32_t M[16];
+        const int mode = (int)(k % 4);
+        if (mode == 0) {
+            for (int i = 0; i < 16; i++)
+                M[i] = (uint32_t)sm64(st);
+            M[15] |= 0x40000000u;
+            M[15] = (sm64(st) & 1) ? (M[15] | 0x80000000u) : (M[15] & 0x7FFFFFFFu);
+        } else {
+            uint32_t s[NL];
+            for (int i = 0; i < NL; i++)
+                s[i] = (uint32_t)sm64(st);
+            s[NL - 1] |= 0x80000000u;
+            if (mode == 3) {
+                const int z = (int)(sm64(st) % NL);
+                for (int i = 0; i < z; i++)
+                    s[i] = (sm64(st) & 1) ? 0u : 0xFFFFFFFFu;
+            }
+            uint32_t sq[16];
+            mul_limbs<NL, NL>(s, s, sq);
+            uint32_t add[16] = {0};
+            int sub1 = 0;
+            switch ((int)(sm64(st) % 5)) {
+            case 0: sub1 = 1; break;                       // s^2 - 1
+            case 1: break;                                 // s^2
+            case 2: add[0] = 1; break;                     // s^2 + 1
+            default: {                                     // s^2 + 2s (- 1)
+                uint64_t c = 0;
+                for (int i = 0; i < NL; i++) {
+                    const uint64_t v = ((uint64_t)s[i] << 1) + c;
+                    add[i] = (uint32_t)v;
+                    c = v >> 32;
+                }
+                add[NL] = (uint32_t)c;
+                sub1 = (int)(sm64(st) & 1);
+            }
+            }
+            uint64_t c = 0;
+            for (int i = 0; i < 16; i++) {
+                c += (uint64_t)sq[i] + add[i];
+                M[i] = (uint32_t)c;
+                c >>= 32;
+            }
+            uint64_t br = (uint64_t)sub1;
+            for (int i = 0; i < 16; i++) {
+                const uint64_t v = (uint64_t)M[i] - br;
+                M[i] = (uint32_t)v;
+                br = (v >> 63) & 1u;
+            }
+            if (!(M[15] >> 30))
+                M[15] |= 0x40000000u;                      // keep the contract's top bits
+        }
+        uint32_t r[NL];
+        bool nz;
+        isqrt_limbs<NL>(M, r, nz);
+        uint32_t sq[16], diff[16], two_r[16] = {0};
+        mul_limbs<NL, NL>(r, r, sq);
+        uint64_t br = 0, c = 0;
+        for (int i = 0; i < 16; i++) {
+            const uint64_t v = (uint64_t)M[i] - sq[i] - br;
+            diff[i] = (uint32_t)v;
+            br = (v >> 63) & 1u;
+        }
+        for (int i = 0; i < NL; i++) {
+            const uint64_t v = ((uint64_t)r[i] << 1) + c;
+            two_r[i] = (uint32_t)v;
+            c = v >> 32;
+        }
+        two_r[NL] = (uint32_t)c;
+        bool zero = true;
+        for (int i = 0; i < 16; i++)
+            zero = zero && diff[i] == 0;
+        if (cmp16(sq, M) > 0 || cmp16(diff, two_r) > 0 || nz == zero)
+            bad++;
+    }
+    return bad;
+}

@@ -143,3 +143,13 @@ def test_two_term_add_is_the_fma_paths_add(emu, oracle, fmt):
             want, wf = oracle_elem(oracle, fmt, 2 if sub else 1, rnd, a, None, c)
             got, gf = emu_elem(emu, fmt, 259 if sub else 258, rnd, a, None, c)
             assert np.array_equal(got, want) and gf == wf, (fmt, rnd, sub)
+
+
+def test_root_digit_recurrence_exact(emu):
+    """The square root's digit recurrence against the integer root's
+    definition on two million radicands (random, and squares of random roots
+    with offsets that put its digit estimates on integer boundaries): the
+    estimates may overshoot and step back, never undershoot."""
+    emu.emu_isqrt_check.restype = ctypes.c_size_t
+    emu.emu_isqrt_check.argtypes = [ctypes.c_uint64, ctypes.c_size_t]
+    assert emu.emu_isqrt_check(2024, 2_000_000) == 0
