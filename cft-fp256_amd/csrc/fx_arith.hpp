@@ -496,7 +496,11 @@ __device__ __forceinline__ void round_fixed(int sign, const uint32_t (&S)[N], in
 // then at T-1, T or T+1 unless the terms were within a binade of each other
 // and of opposite sign (deep cancellation), which takes a rare branch that
 // normalises by a full shift.
-template <int NL>
+//
+// ONE_ROUND: the rare branch and the common one share a single rounding (the
+// elementwise kernel, where waves often take both); otherwise each has its
+// own, which the sequencer's kernels hold in fewer registers.
+template <int NL, bool ONE_ROUND = false>
 __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const uint32_t (&xb)[NL],
                                          const uint32_t (&xc)[NL], int rnd, uint32_t (&out)[NL],
                                          uint32_t &flags)
@@ -532,36 +536,52 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     const int sp = ua.sign ^ ub.sign;
 
     if (ua.nan || ub.nan || uc.nan || ua.inf || ub.inf || uc.inf || ua.zero || ub.zero) {
-        if (ua.nan || ub.nan || uc.nan) {
-            enc_qnan<NL>(out);
-            flags |= (ua.snan || ub.snan || uc.snan) ? FL_INV : 0u;
-        } else if (ua.inf || ub.inf) {
-            if (ua.zero || ub.zero || (uc.inf && uc.sign != sp)) {
-                enc_qnan<NL>(out);
-                flags |= FL_INV;
-            } else {
-                enc_inf<NL>(out, sp);
-            }
-        } else if (uc.inf) {
-            enc_inf<NL>(out, uc.sign);
-        } else if (uc.zero) {
-            enc_zero<NL>(out, uc.sign == sp ? uc.sign : (rnd == RDN));
-        } else {
-            // 0*b + c is c, bits and all (the steered c: SUB's flipped sign)
+        // the special result by selects, in the same order: a NaN operand
+        // gives qNaN (INVALID for a signalling one); an infinite factor gives
+        // qNaN and INVALID times zero or against an opposite infinite c, else
+        // an infinity; then an infinite c; a zero product gives the zero of
+        // the sum's sign rule when c is zero, else c itself, bits and all
+        // (the steered c: SUB's flipped sign)
+        const bool anynan = ua.nan || ub.nan || uc.nan;
+        const bool pinf = ua.inf || ub.inf;
+        const bool badprod = pinf && (ua.zero || ub.zero || (uc.inf && uc.sign != sp));
+        const bool qnan = anynan || badprod;
+        const bool inf = !qnan && (pinf || uc.inf);
+        const bool zero = !qnan && !inf && uc.zero;
+        const bool isc = !qnan && !inf && !zero;
+        const uint32_t zsign = uc.sign == sp ? (uint32_t)uc.sign : (rnd == RDN ? 1u : 0u);
+        const uint32_t top_inf = ((uint32_t)(pinf ? sp : uc.sign) << 31) | (Fm::EMASK << Fm::TOPF);
+        const uint32_t top_nan = (Fm::EMASK << Fm::TOPF) | (1u << (Fm::TOPF - 1));
+        const uint32_t top_c = xc[NL - 1] ^ (op == OP_SUB ? 0x80000000u : 0u);
 #pragma unroll
-            for (int i = 0; i < NL - 1; i++)
-                out[i] = xc[i];
-            out[NL - 1] = xc[NL - 1] ^ (op == OP_SUB ? 0x80000000u : 0u);
-        }
+        for (int i = 0; i < NL - 1; i++)
+            out[i] = isc ? xc[i] : 0u;
+        out[NL - 1] = qnan ? top_nan : inf ? top_inf : zero ? (zsign << 31) : top_c;
+        flags |= (anynan ? (ua.snan || ub.snan || uc.snan) : badprod) ? FL_INV : 0u;
         return;
     }
     // MUL rounds from the product's top bit, which needs normal factors.  FMA,
-    // ADD and SUB leave a subnormal significand raw: each term is placed by
-    // its lsb exponent, so a subnormal one merely has its leading bit below
-    // T, and the sum then takes the general normalisation below.
-    if (op == OP_MUL && (ua.sub || ub.sub)) {
-        normalise<NL>(ua);
-        normalise<NL>(ub);
+    // ADD and SUB may leave a subnormal significand raw: each term is placed
+    // by its lsb exponent, so a subnormal one merely has its leading bit
+    // below T, and the sum then takes the general normalisation below.  That
+    // renormalises all 2NL limbs of the sum, so a subnormal operand whose
+    // term can be the larger one (by the exponents of bit P-1 of c and bit
+    // 2P-2 of the product, within two binades) is normalised first, NL limbs:
+    // the sum's rare branch is then left to cancellations.
+    if (op == OP_MUL) {
+        if (ua.sub || ub.sub) {
+            normalise<NL>(ua);
+            normalise<NL>(ub);
+        }
+    } else if (ua.sub || ub.sub || uc.sub) {
+        const int vp = ua.e + ub.e + 2 * P - 2, vc = uc.e + P - 1;
+        const bool pmay = !uc.zero ? vp + 2 > vc : true, cmay = !uc.zero && vc + 2 > vp;
+        if (ua.sub && pmay)
+            normalise<NL>(ua);
+        if (ub.sub && pmay)
+            normalise<NL>(ub);
+        if (uc.sub && cmay)
+            normalise<NL>(uc);
     }
 
     // the exact product, leading bit at 2P-2 or 2P-1 for normal factors
@@ -657,6 +677,8 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     const bool tm = (S[(T - 1) / 32] >> ((T - 1) % 32)) & 1u;
     const bool neg = (S[WN - 1] >> 31) & 1u;     // only when d == 0, subtracting
     int sign = bsign;
+    int M = t1 ? T + 1 : (t0 ? T : T - 1);
+    int e_norm = VEb + (M - T);
     if (neg || (!t1 && !t0 && !tm)) {
         // rare: equal leading exponents cancelled, the smaller term won, or
         // the larger term was subnormal (a raw significand, or a product
@@ -671,8 +693,12 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
             return;
         }
         shl<WN>(S, (uint32_t)(T + 1 - (L - 1)));  // leading bit to T+1
-        round_fixed<NL, WN, T>(sign, S, T + 1, VEb + (L - 1) - T, rnd, out, flags);
-        return;
+        M = T + 1;
+        e_norm = VEb + (L - 1) - T;
+        if constexpr (!ONE_ROUND) {
+            round_fixed<NL, WN, T>(sign, S, M, e_norm, rnd, out, flags);
+            return;
+        }
     }
     // Whenever the jam bit was set, the sum's rounding position lies at bit 2
     // or above: the leading bit is at T-1 or higher with both terms normal;
@@ -680,8 +706,7 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     // larger one, the result is below 2^emin and rounds at C's bit 0, at
     // T-P+1; and a product with a subnormal factor that is the larger term
     // only drops bits of a C at least 2^(T-P+1) times smaller.
-    const int M = t1 ? T + 1 : (t0 ? T : T - 1);
-    round_fixed<NL, WN, T>(sign, S, M, VEb + (M - T), rnd, out, flags);
+    round_fixed<NL, WN, T>(sign, S, M, e_norm, rnd, out, flags);
 }
 
 // a + c, or a - c when neg_c: the value, bits and flags of fma_core's ADD /
@@ -798,7 +823,7 @@ __device__ __forceinline__ void fma_op(const uint32_t (&xa)[NL], const uint32_t 
                                        const uint32_t (&xc)[NL], int rnd, uint32_t (&out)[NL],
                                        uint32_t &flags)
 {
-    fma_core<NL>(OP, xa, xb, xc, rnd, out, flags);
+    fma_core<NL, true>(OP, xa, xb, xc, rnd, out, flags);
 }
 
 } // namespace fx
