@@ -569,7 +569,10 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     // 2P-2 of the product, within two binades) is normalised first, NL limbs:
     // the sum's rare branch is then left to cancellations.
     if (op == OP_MUL) {
-        if (ua.sub || ub.sub) {
+        // unless even the product's bit 2P-1 lies P+1 or more places below
+        // 2^emin: then it rounds to +-0 or the least subnormal whatever its
+        // bits, and round_pack_norm takes that from the exponent alone
+        if ((ua.sub || ub.sub) && ua.e + ub.e + 2 * P - 1 > Fm::EMIN - P - 1) {
             normalise<NL>(ua);
             normalise<NL>(ub);
         }

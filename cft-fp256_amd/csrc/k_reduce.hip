@@ -180,9 +180,10 @@ __global__ __launch_bounds__(256, 4) void k_reduce(const uint32_t *__restrict__ 
                     v3[NL - 1] &= 0x7FFFFFFFu;
                 }
                 if constexpr (LEAF == LEAF_MUL || LEAF == LEAF_SQ) {
-                    // the products in place, one multiply body: each turn
-                    // multiplies v0 and rotates it to the back
-#pragma unroll 1
+                    // the products in place: each step multiplies v0 and
+                    // rotates it to the back (unrolled, the rotation is only
+                    // register naming)
+#pragma unroll
                     for (uint32_t k = 0; k < 4; k++) {
                         uint32_t r[NL];
                         if (k < cnt) {
