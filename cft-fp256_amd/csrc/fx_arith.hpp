@@ -497,10 +497,15 @@ __device__ __forceinline__ void round_fixed(int sign, const uint32_t (&S)[N], in
 // and of opposite sign (deep cancellation), which takes a rare branch that
 // normalises by a full shift.
 //
-// ONE_ROUND: the rare branch and the common one share a single rounding (the
-// elementwise kernel, where waves often take both); otherwise each has its
-// own, which the sequencer's kernels hold in fewer registers.
-template <int NL, bool ONE_ROUND = false>
+// ELEM: the elementwise kernels' variant.  There nearly every wavefront holds
+// special and subnormal lanes, so a subnormal operand whose term can be the
+// larger is normalised up front (N limbs, not the sum's 2N) and the rare and
+// common sums share one rounding.  The sequencer's kernels, whose wavefronts
+// rarely hold a subnormal, keep the leaner control flow: the rare sum has a
+// rounding of its own and subnormal terms go to the sum's normalisation,
+// which their kernels hold in fewer registers and a lone wavefront runs in
+// fewer instructions.
+template <int NL, bool ELEM = false>
 __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const uint32_t (&xb)[NL],
                                          const uint32_t (&xc)[NL], int rnd, uint32_t (&out)[NL],
                                          uint32_t &flags)
@@ -564,10 +569,11 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
     // ADD and SUB may leave a subnormal significand raw: each term is placed
     // by its lsb exponent, so a subnormal one merely has its leading bit
     // below T, and the sum then takes the general normalisation below.  That
-    // renormalises all 2NL limbs of the sum, so a subnormal operand whose
-    // term can be the larger one (by the exponents of bit P-1 of c and bit
-    // 2P-2 of the product, within two binades) is normalised first, NL limbs:
-    // the sum's rare branch is then left to cancellations.
+    // renormalises all 2NL limbs of the sum, so in the elementwise variant a
+    // subnormal operand whose term can be the larger one (by the exponents of
+    // bit P-1 of c and bit 2P-2 of the product, within two binades) is
+    // normalised first, NL limbs: the sum's rare branch is then left to
+    // cancellations.
     if (op == OP_MUL) {
         // unless even the product's bit 2P-1 lies P+1 or more places below
         // 2^emin: then it rounds to +-0 or the least subnormal whatever its
@@ -576,7 +582,7 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
             normalise<NL>(ua);
             normalise<NL>(ub);
         }
-    } else if (ua.sub || ub.sub || uc.sub) {
+    } else if (ELEM && (ua.sub || ub.sub || uc.sub)) {
         const int vp = ua.e + ub.e + 2 * P - 2, vc = uc.e + P - 1;
         const bool pmay = !uc.zero ? vp + 2 > vc : true, cmay = !uc.zero && vc + 2 > vp;
         if (ua.sub && pmay)
@@ -698,7 +704,7 @@ __device__ __forceinline__ void fma_core(int op, const uint32_t (&xa)[NL], const
         shl<WN>(S, (uint32_t)(T + 1 - (L - 1)));  // leading bit to T+1
         M = T + 1;
         e_norm = VEb + (L - 1) - T;
-        if constexpr (!ONE_ROUND) {
+        if constexpr (!ELEM) {
             round_fixed<NL, WN, T>(sign, S, M, e_norm, rnd, out, flags);
             return;
         }

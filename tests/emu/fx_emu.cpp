@@ -30,6 +30,11 @@ static void one(int op, int rnd, const uint32_t *a, const uint32_t *b, const uin
     case 257: sqrt_op<NL>(xa, rnd, o, fl); break;
     case 258: add_core<NL>(xa, xc, false, rnd, o, fl); break;   // the two-term ADD path
     case 259: add_core<NL>(xa, xc, true, rnd, o, fl); break;    // and SUB
+    // the sequencer's variant of the FMA core (FMA, ADD, SUB, MUL)
+    case 260: fma_core<NL, false>(OP_FMA, xa, xb, xc, rnd, o, fl); break;
+    case 261: fma_core<NL, false>(OP_ADD, xa, xb, xc, rnd, o, fl); break;
+    case 262: fma_core<NL, false>(OP_SUB, xa, xb, xc, rnd, o, fl); break;
+    case 263: fma_core<NL, false>(OP_MUL, xa, xb, xc, rnd, o, fl); break;
     default: misc_op<NL>(op, xa, xb, xc, o, fl); break;
     }
     memcpy(out, o, sizeof o);
