@@ -80,8 +80,8 @@ VALU_4CYCLE_WAVE_INSNS = 256 * 4 * 2.4e9 / 4
 # cycles a SIMD spends a VALU instruction there, GRBM_GUI_ACTIVE / 8 XCDs
 # over SQ_INSTS_VALU / 1,024 SIMDs: profiles/valu_issue_rates.md); the exact
 # product's 32x32->64 multiply-accumulates an element (8 x 8 limbs)
-FMA_VALU_PER_ELEM = 854.6
-FMA_CYCLES_PER_VALU = 4.16
+FMA_VALU_PER_ELEM = 676.7
+FMA_CYCLES_PER_VALU = 5.17
 FMA_MACS = 64
 
 
@@ -399,7 +399,7 @@ def rk4_section(dev, lib, args, threads, ref):
     from cftamd import tune_batch
     prog.prepare(0x7F)
     s_tune = np.maximum(1, steps // 100).astype(np.uint32)
-    knobs, trials = tune_batch(prog, state, s_tune, bank=bank)
+    knobs, trials = tune_batch(prog, state, s_tune, bank=bank, repeats=2)
     log(f"bench: rk4 tuned {knobs} ({len(trials)} trials, {trials[0][0]:.4f} .. "
         f"{trials[-1][0]:.4f} s)")
     # warm the graph on this exact launch shape: a pass over a copy of the
