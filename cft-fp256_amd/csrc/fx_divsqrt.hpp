@@ -108,14 +108,14 @@ __device__ __forceinline__ bool plain_operand(const uint32_t (&x)[NL])
     return ef != 0 && ef != Fm::EMASK;
 }
 
-// a / b for finite nonzero operands, unpacked and normalised.
+// The quotient of finite nonzero operands, unpacked and normalised, as
+// round_pack_norm takes it: K (P bits), guard, sticky and K's exponent.
 template <int NL>
-__device__ __forceinline__ void div_finite(const Unp<NL> &ua, const Unp<NL> &ub, int rnd,
-                                           uint32_t (&out)[NL], uint32_t &flags)
+__device__ __forceinline__ void div_quotient(const Unp<NL> &ua, const Unp<NL> &ub,
+                                             uint32_t (&K)[NL], bool &g, bool &s, int &e_norm)
 {
     using Fm = F<NL>;
     constexpr int P = Fm::P;
-    const int sq = ua.sign ^ ub.sign;
     // Q = floor(ma * 2^k / mb) with exactly P+3 bits: k = P+2 when ma >= mb
     bool ge = true;
 #pragma unroll
@@ -143,22 +143,35 @@ __device__ __forceinline__ void div_finite(const Unp<NL> &ua, const Unp<NL> &ub,
     bool rem_nz;
     long_div<NL>(U, D, Qd, rem_nz);
     // Q has P+3 bits: K = Q >> 3, guard bit 2, sticky bits 1..0 and the rest
-    uint32_t K[NL];
 #pragma unroll
     for (int i = 0; i < NL; i++)
         K[i] = alignbit(i + 1 < NL ? Qd[i + 1] : 0u, Qd[i], 3);
-    const bool g = (Qd[0] >> 2) & 1u;
-    const bool s = (Qd[0] & 3u) != 0 || rem_nz;
-    round_pack_norm<NL>(sq, K, g, s, ua.e - ub.e - k + P + 2, rnd, out, flags);
+    g = (Qd[0] >> 2) & 1u;
+    s = (Qd[0] & 3u) != 0 || rem_nz;
+    e_norm = ua.e - ub.e - k + P + 2;
+}
+
+// a / b for finite nonzero operands, unpacked and normalised.
+template <int NL>
+__device__ __forceinline__ void div_finite(const Unp<NL> &ua, const Unp<NL> &ub, int rnd,
+                                           uint32_t (&out)[NL], uint32_t &flags)
+{
+    uint32_t K[NL];
+    bool g, s;
+    int e_norm;
+    div_quotient<NL>(ua, ub, K, g, s, e_norm);
+    round_pack_norm<NL>(ua.sign ^ ub.sign, K, g, s, e_norm, rnd, out, flags);
 }
 
 template <int NL>
 __device__ __forceinline__ void div_op(const uint32_t (&xa)[NL], const uint32_t (&xb)[NL], int rnd,
                                        uint32_t (&out)[NL], uint32_t &flags)
 {
+    using Fm = F<NL>;
+    constexpr int P = Fm::P;
     Unp<NL> ua, ub;
-    unpack<NL>(xa, ua);
-    unpack<NL>(xb, ub);
+    unpack_raw<NL>(xa, ua);
+    unpack_raw<NL>(xb, ub);
     const int sq = ua.sign ^ ub.sign;
     if (ua.nan || ub.nan) {
         enc_qnan<NL>(out);
@@ -177,7 +190,30 @@ __device__ __forceinline__ void div_op(const uint32_t (&xa)[NL], const uint32_t 
         return;
     }
     if (ua.zero) { enc_zero<NL>(out, sq); return; }
-    div_finite<NL>(ua, ub, rnd, out, flags);
+    // A subnormal dividend (below 2^emin) over a normal divisor of 2^P or
+    // more (lsb exponent 1 or more) is a quotient P+1 or more places below
+    // 2^emin; a normal dividend of 4 or more (leading exponent 2 or more)
+    // over a subnormal divisor is one of 2^(emax+1) or more.  Either rounds
+    // by its exponent alone (to +-0 or the least subnormal, or to an infinity
+    // or the largest finite number), so those lanes skip the normalisation
+    // and the division and give round_pack_norm just an exponent that says
+    // so (K is then read for nothing but a carry).
+    const bool tiny = ua.sub && !ub.sub && ub.e >= 1;
+    const bool huge = ub.sub && !ua.sub && ua.e + Fm::MANW >= 2;
+    uint32_t K[NL];
+    bool g = false, s = true;
+    int e_norm;
+    if (tiny || huge) {
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+            K[i] = ua.m[i];
+        e_norm = tiny ? Fm::EMIN - P - 1 : Fm::EMAX + 1;
+    } else {
+        normalise<NL>(ua);
+        normalise<NL>(ub);
+        div_quotient<NL>(ua, ub, K, g, s, e_norm);
+    }
+    round_pack_norm<NL>(sq, K, g, s, e_norm, rnd, out, flags);
 }
 
 // r = floor(sqrt(M)), M 2*NL limbs with its top bit at 64NL-1 or 64NL-2;
