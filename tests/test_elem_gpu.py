@@ -242,3 +242,21 @@ def test_resident_buffers(device, oracle):
     assert np.array_equal(d.host, want2)
     for buf in bufs:
         buf.free()
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_exponent_settled_quotients_and_products(device, oracle, fmt):
+    """Division and multiplication at the thresholds where a result is
+    settled by its exponent alone, both sides of them: bits against the
+    oracle, and flags per exponent class of the operands."""
+    from test_oracle import _edges
+    rng = np.random.default_rng(700 + fmt)
+    esz = 4 << fmt
+    for kind, op in (("div", 256), ("mul", 3)):
+        a, b = _edges(rng, fmt, 4000, kind)
+        for rnd in range(5):
+            want, wf = oracle_elem(oracle, fmt, op, rnd, a, b, None)
+            d, fl = gpu(device, fmt, op, rnd, a, b, None)
+            bad = np.nonzero((d.reshape(-1, esz) != want.reshape(-1, esz)).any(axis=1))[0]
+            assert bad.size == 0, f"fmt {fmt} {kind} rnd {rnd}: elements {bad[:8]} differ"
+            assert fl == wf, (fmt, kind, rnd)

@@ -77,6 +77,50 @@ def _next_to(rng, fmt, n):
             np.frombuffer(b"".join(bb), dtype=np.uint8).copy())
 
 
+def _edges(rng, fmt, n, kind):
+    """(a, b) at the thresholds where division and multiplication settle a
+    result by its exponent alone.  kind "div": a subnormal dividend over a
+    normal divisor whose lsb exponent is -3..3 (from 1 up, the quotient is
+    P+1 or more places below 2^emin), and a normal dividend of leading
+    exponent -1..4 over a subnormal divisor (from 2 up, the quotient is past
+    the overflow threshold), a tenth of the pairs both subnormal.  kind
+    "mul": a subnormal factor times a normal one whose lsb exponent is
+    -2P-4 .. -2P+2 (at -2P-1 and below the product's top bit lies P+1 or
+    more places below 2^emin).  Random signs and significands."""
+    ew, mw, w = {0: (8, 23, 32), 1: (11, 52, 64), 2: (15, 112, 128), 3: (19, 236, 256)}[fmt]
+    esz, bias, p = w // 8, (1 << (ew - 1)) - 1, mw + 1
+
+    def enc(s, e, m):
+        return ((s << (w - 1)) | (e << mw) | m).to_bytes(esz, "little")
+
+    def frac():
+        return int.from_bytes(rng.bytes(esz), "little") % (1 << mw)
+
+    def sub():
+        m = frac() >> int(rng.integers(0, mw))
+        return m or 1
+
+    aa, bb = [], []
+    for i in range(n):
+        sa, sb = (int(x) for x in rng.integers(0, 2, size=2))
+        if kind == "mul":
+            # lsb exponent of b = biased - bias - mw
+            e = bias + mw + (-2 * p - 1) + int(rng.integers(-3, 4))
+            x, y = enc(sa, 0, sub()), enc(sb, e, frac())
+            if i % 2:
+                x, y = y, x
+        elif i % 10 == 0:
+            x, y = enc(sa, 0, sub()), enc(sb, 0, sub())
+        elif i % 2:
+            x, y = enc(sa, 0, sub()), enc(sb, bias + mw + int(rng.integers(-3, 4)), frac())
+        else:
+            x, y = enc(sa, bias + int(rng.integers(-1, 5)), frac()), enc(sb, 0, sub())
+        aa.append(x)
+        bb.append(y)
+    return (np.frombuffer(b"".join(aa), dtype=np.uint8).copy(),
+            np.frombuffer(b"".join(bb), dtype=np.uint8).copy())
+
+
 @pytest.mark.parametrize("fmt", [0, 1, 2, 3])
 def test_oracle_matches_reference_library(oracle, ref, fmt):
     """Every elementwise opcode the engine runs, all modes, 600 cases each."""
