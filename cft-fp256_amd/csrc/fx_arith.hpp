@@ -485,17 +485,19 @@ __device__ __forceinline__ void round_fixed(int sign, const uint32_t (&S)[N], in
 // and 438-468 (ADD = a*1 + c, SUB = a*1 - c, MUL = a*b + (+-0)).  op is
 // uniform across the wave (a kernel constant, or an interpreted instruction).
 //
-// Method.  Subnormal operands are normalised first (one rare branch), so the
-// exact product P = ma*mb has its leading bit at 2P-2 or 2P-1.  The exact sum
-// is formed in a window of 2NL limbs in which the LARGER term - by leading-bit
-// exponent - sits with its leading bit at a fixed bit T = 64NL-7, and the
+// Method.  For normal factors the exact product P = ma*mb has its leading
+// bit at 2P-2 or 2P-1.  The exact sum is formed in a window of 2NL limbs in
+// which the LARGER term - by the exponent of bit 2P-2 of the product and of
+// bit P-1 of c - sits with that bit at a fixed bit T = 64NL-7, and the
 // smaller term is placed by one variable right shift, every bit shifted past
 // the bottom jammed into bit 0 (which no real bit of either term occupies).
 // Jamming is exact in effect: the representative lies in the same open
 // interval between consecutive rounding boundaries.  The sum's leading bit is
 // then at T-1, T or T+1 unless the terms were within a binade of each other
-// and of opposite sign (deep cancellation), which takes a rare branch that
-// normalises by a full shift.
+// and of opposite sign (deep cancellation) or the larger term holds a
+// subnormal significand (its leading bit below T), which take a rare branch
+// that normalises the sum by a full shift.  (MUL normalises its factors
+// instead: it rounds from the product's top bit.)
 //
 // ELEM: the elementwise kernels' variant.  There nearly every wavefront holds
 // special and subnormal lanes, so a subnormal operand whose term can be the
