@@ -403,10 +403,18 @@ def rk4_section(dev, lib, args, threads, ref):
     log(f"bench: rk4 tuned {knobs} ({len(trials)} trials, {trials[0][0]:.4f} .. "
         f"{trials[-1][0]:.4f} s)")
     # warm the graph on this exact launch shape: a pass over a copy of the
-    # ensemble with one step a lane, then the timed pass over the real steps
+    # ensemble with one step a lane, one untimed pass over the real steps (a
+    # process's first seconds-long launch runs ~10% slower: the card's
+    # clocks), then the timed pass
     sbuf.host[:] = state
     sbuf.to_device()
     prog.run_batch(sbuf, np.ones_like(steps), bank=bbuf, graph=True, **knobs)
+    sbuf.from_device()
+    sbuf.host[:] = state
+    sbuf.to_device()
+    t0 = time.perf_counter()
+    prog.run_batch(sbuf, steps, bank=bbuf, graph=True, **knobs)
+    first_s = time.perf_counter() - t0
     sbuf.from_device()
     sbuf.host[:] = state
     sbuf.to_device()
@@ -424,6 +432,7 @@ def rk4_section(dev, lib, args, threads, ref):
         "steps_min": int(steps.min()), "steps_max": int(steps.max()),
         "persistent_s": round(secs, 4), "kernel_s": round(kms.value / 1e3, 4),
         "steps_per_s": round(total / secs, 1), "alu_ops_per_s": round(53 * total / secs, 1),
+        "untimed_first_pass_s": round(first_s, 4),
         "flags": fl, "status": st, "launches": int(lib.cftx_batch_launches(dev.handle)),
         "lanes_per_instance": int(lib.cftx_batch_width(dev.handle)),
         "tuned_knobs": knobs, "tuning_sample": "steps // 100 (at least 1)",
