@@ -2,20 +2,13 @@
 // one lane per element.  Replace the reference's composed routes (host/src/
 // divsqrt.c: ~25-30 elementwise passes, or a Newton program) with one pass:
 // the contract is "correctly rounded", so the bits and flags are the same.
+#include "elem_io.hpp"
 #include "fx_divsqrt.hpp"
 #include "kernels.h"
 
 using namespace fx;
 
 namespace {
-
-template <int NL>
-__device__ __forceinline__ void ld(const uint32_t *__restrict__ p, size_t i, uint32_t (&x)[NL])
-{
-#pragma unroll
-    for (int k = 0; k < NL; k++)
-        x[k] = p[i * NL + k];
-}
 
 template <int NL, int WHICH>
 __global__ __launch_bounds__(256) void k_divsqrt(const uint32_t *__restrict__ a,
@@ -28,16 +21,14 @@ __global__ __launch_bounds__(256) void k_divsqrt(const uint32_t *__restrict__ a,
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         uint32_t xa[NL], xb[NL], o[NL];
-        ld<NL>(a, i, xa);
+        load_el<NL>(a, i, xa);
         if (WHICH == 0) {
-            ld<NL>(b, i, xb);
+            load_el<NL>(b, i, xb);
             div_op<NL>(xa, xb, rnd, o, fl);
         } else {
             sqrt_op<NL>(xa, rnd, o, fl);
         }
-#pragma unroll
-        for (int k = 0; k < NL; k++)
-            d[i * NL + k] = o[k];
+        store_el<NL>(d, i, o);
     }
     // the wavefront's OR of its lanes' flags, one atomic into its flag slot
     // for bits not yet set (no LDS: see flush_flags in k_elem.hip)

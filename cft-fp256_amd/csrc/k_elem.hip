@@ -2,13 +2,14 @@
 // (FMA, ADD, SUB, MUL) and the sign/compare/integer groups, binary32 ..
 // binary256.  The caller's arrays are dense little-endian elements (host/
 // include/cft.h:1166-1209); each lane owns one element at a time, loads it
-// with 16-byte vector loads and keeps it in VGPRs as 32-bit limbs.
+// with 16-byte vector loads (elem_io.hpp) and keeps it in VGPRs as 32-bit limbs.
 //
 // Replaces the reference's k_run / k_run256 / k_run128 (host/src/
 // backend_hip.cpp:484-665).  Launch shape: a grid-stride loop, launched by
 // the library with one workgroup for every 256 elements; each wavefront ORs
 // its flags into one of the call's flag slots (kernels.h) with at most one
 // atomic.
+#include "elem_io.hpp"
 #include "fx_arith.hpp"
 #include "fx_misc.hpp"
 #include "kernels.h"
@@ -16,43 +17,6 @@
 using namespace fx;
 
 namespace {
-
-template <int NL>
-__device__ __forceinline__ void load_el(const uint32_t *__restrict__ p, size_t i,
-                                        uint32_t (&x)[NL])
-{
-    if constexpr (NL == 8) {
-        const uint4 *q = reinterpret_cast<const uint4 *>(p + i * 8);
-        const uint4 v0 = q[0], v1 = q[1];
-        x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
-        x[4] = v1.x; x[5] = v1.y; x[6] = v1.z; x[7] = v1.w;
-    } else if constexpr (NL == 4) {
-        const uint4 v0 = reinterpret_cast<const uint4 *>(p)[i];
-        x[0] = v0.x; x[1] = v0.y; x[2] = v0.z; x[3] = v0.w;
-    } else if constexpr (NL == 2) {
-        const uint2 v0 = reinterpret_cast<const uint2 *>(p)[i];
-        x[0] = v0.x; x[1] = v0.y;
-    } else {
-        x[0] = p[i];
-    }
-}
-
-template <int NL>
-__device__ __forceinline__ void store_el(uint32_t *__restrict__ p, size_t i,
-                                         const uint32_t (&x)[NL])
-{
-    if constexpr (NL == 8) {
-        uint4 *q = reinterpret_cast<uint4 *>(p + i * 8);
-        q[0] = make_uint4(x[0], x[1], x[2], x[3]);
-        q[1] = make_uint4(x[4], x[5], x[6], x[7]);
-    } else if constexpr (NL == 4) {
-        reinterpret_cast<uint4 *>(p)[i] = make_uint4(x[0], x[1], x[2], x[3]);
-    } else if constexpr (NL == 2) {
-        reinterpret_cast<uint2 *>(p)[i] = make_uint2(x[0], x[1]);
-    } else {
-        p[i] = x[0];
-    }
-}
 
 // One wavefront's flags into the call's word: the OR over its lanes by one
 // ballot a flag bit, then one global atomic from its first active lane, and
