@@ -210,14 +210,48 @@ __device__ __forceinline__ void unpack_raw(const uint32_t (&x)[NL], Unp<NL> &u)
 
 // A subnormal's significand shifted so its leading bit is at P-1, its
 // exponent lowered to match; any other value is left alone.
+//
+// Whole limbs first: select levels of 4, 2 and 1 limbs, each taken while
+// the value stays below 2^(P-1-32t), so that the leading bit ends 1 to 32
+// places below P-1 (it starts at P-2 or lower, and NL-1 limbs bring even bit
+// 0 that far).  Then one funnel shift by those v places, a limb at a time:
+// v = 32 is the funnel's shift count 0, which yields the lower limb, the
+// shift by a whole limb that v = 32 asks for.  No bit length and no second
+// shift array, as a general left shift would need.
 template <int NL>
 __device__ __forceinline__ void normalise(Unp<NL> &u)
 {
     using Fm = F<NL>;
     if (u.sub) {
-        const int L = bitlen<NL>(u.m);
-        shl<NL>(u.m, (uint32_t)(Fm::P - L));
-        u.e -= Fm::P - L;
+        int k = 0;
+#pragma unroll
+        for (int t = 4; t >= 1; t >>= 1) {
+            if (t > NL - 1)
+                continue;
+            // bits 32(NL-1-t) + TOPF and up all clear: below 2^(P-1-32t)
+            uint32_t hi = u.m[NL - 1 - t] >> Fm::TOPF;
+#pragma unroll
+            for (int i = NL - t; i < NL; i++)
+                hi |= u.m[i];
+            const bool s = hi == 0;
+#pragma unroll
+            for (int i = NL - 1; i >= 0; i--)
+                u.m[i] = s ? (i >= t ? u.m[i - t] : 0u) : u.m[i];
+            k += s ? t : 0;
+        }
+        // the leading bit: limb NL-1 below TOPF, or limb NL-2 at TOPF or up
+        int v;
+        if constexpr (NL == 1) {
+            v = Fm::TOPF - 31 + clz32(u.m[0]);
+        } else {
+            const uint32_t top = u.m[NL - 1];
+            v = top ? Fm::TOPF - 31 + clz32(top) : Fm::TOPF + 1 + clz32(u.m[NL - 2]);
+        }
+        const uint32_t sh = 32u - (uint32_t)v;       // 0 when v = 32
+#pragma unroll
+        for (int i = NL - 1; i >= 0; i--)
+            u.m[i] = alignbit(u.m[i], i ? u.m[i - 1] : 0u, sh);
+        u.e -= 32 * k + v;
         u.sub = false;
     }
 }

@@ -31,7 +31,7 @@ __device__ __forceinline__ void long_div(uint32_t (&U)[2 * NL + 1], const uint32
     double dtop = (double)D[NL - 1] * 4294967296.0;
     if constexpr (NL >= 2)
         dtop += (double)D[NL - 2];
-    const double rinv = 1.0 / dtop;
+    const double rinv = recip_mid(dtop);         // dtop in [2^63, 2^64)
 #pragma unroll
     for (int j = NL - 1; j >= 0; j--) {
         // estimate from U[j+NL], U[j+NL-1], U[j+NL-2] against D's top two.
@@ -116,11 +116,17 @@ __device__ __forceinline__ void div_quotient(const Unp<NL> &ua, const Unp<NL> &u
 {
     using Fm = F<NL>;
     constexpr int P = Fm::P;
-    // Q = floor(ma * 2^k / mb) with exactly P+3 bits: k = P+2 when ma >= mb
-    bool ge = true;
+    // Q = floor(ma * 2^k / mb) with exactly P+3 bits: k = P+2 when ma >= mb,
+    // which is no borrow out of ma - mb (one borrow chain, the differences
+    // unused)
+    uint32_t br = 0;
 #pragma unroll
-    for (int i = 0; i < NL; i++)
-        ge = ua.m[i] != ub.m[i] ? ua.m[i] > ub.m[i] : ge;
+    for (int i = 0; i < NL; i++) {
+        uint32_t bo;
+        (void)subc(ua.m[i], ub.m[i], br, &bo);
+        br = bo;
+    }
+    const bool ge = br == 0;
     const int k = ge ? P + 2 : P + 3;
     constexpr int SD = 32 * NL - P;              // divisor normalisation
     uint32_t D[NL];
@@ -237,7 +243,7 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
     {
         const uint64_t g = ((uint64_t)M[2 * NL - 1] << 32) | M[2 * NL - 2];
         const double gd = __builtin_fma((double)M[2 * NL - 1], 0x1p32, (double)M[2 * NL - 2]);
-        uint32_t d = (uint32_t)__builtin_fmin(sqrt(gd), 4294967295.0);
+        uint32_t d = (uint32_t)__builtin_fmin(sqrt_mid(gd), 4294967295.0);   // gd >= 2^62
         uint64_t sq = (uint64_t)d * d;
         const bool over = sq > g;
         const uint64_t up = sq + 2ull * d + 1ull;        // (d+1)^2
@@ -306,7 +312,7 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         // it never undershoots.
         double dd;
         if (k == 1) {
-            rinv = 1.0 / __builtin_fma((double)V[1], 0x1p64, (double)V[0] * 0x1p32);
+            rinv = recip_mid(__builtin_fma((double)V[1], 0x1p64, (double)V[0] * 0x1p32));
             const double qd = Rs * rinv;
             dd = __builtin_fma(-qd, qd * rinv, qd) + 0x1p-15;
         } else {
@@ -317,7 +323,11 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
         uint32_t d = (uint32_t)__builtin_fmin(dd, 4294967295.0);   // dd > 0
         // R -= T, T = (V*B + d) * d: the limbs of V*B + d are [d, V0 .. Vk];
         // their products with d are formed independently, assembled by one
-        // add-with-carry chain and subtracted by one borrow chain
+        // add-with-carry chain and subtracted by one borrow chain.  R lies
+        // below 2^(32(k+2)+1) (the last remainder, under 2r+1 < 2^(32k+1),
+        // brought down two limbs) and T below 2^(32(k+3)), so limbs k+3 up
+        // are zero before and after: the chains stop at limb k+2, its borrow
+        // out standing for R's sign (R = R[0..k+2] - br 2^(32(k+3))).
         uint32_t pl[NL + 1], ph[NL + 1];
 #pragma unroll
         for (int i = 0; i <= k + 1; i++) {
@@ -336,14 +346,12 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
                 R[i] = subc(R[i], ti, br, &bo);
                 br = bo;
             }
-#pragma unroll
-            for (int i = k + 2; i < NL + 2; i++) {
-                uint32_t bo;
-                R[i] = subc(R[i], i == k + 2 ? ph[k + 1] + c : 0u, br, &bo);
-                br = bo;
-            }
+            uint32_t bo;
+            R[k + 2] = subc(R[k + 2], ph[k + 1] + c, br, &bo);
+            br = bo;
         }
-        // while R < 0: R += V*B + 2d - 1 (= T(d) - T(d - 1)); d--
+        // while R < 0: R += V*B + 2d - 1 (= T(d) - T(d - 1)); d--.  A carry
+        // out of limb k+2 cancels the borrow: R is then no longer negative.
         while (br) {
             uint32_t Sv[NL + 2];
             {
@@ -351,7 +359,7 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
                 Sv[0] = (uint32_t)low;
                 uint64_t c = low >> 32;
 #pragma unroll
-                for (int i = 1; i < NL + 2; i++) {
+                for (int i = 1; i <= k + 2; i++) {
                     c += (i - 1 <= k) ? V[i - 1] : 0u;
                     Sv[i] = (uint32_t)c;
                     c >>= 32;
@@ -359,7 +367,7 @@ __device__ __forceinline__ void isqrt_limbs(const uint32_t (&M)[2 * NL], uint32_
             }
             uint32_t c = 0;
 #pragma unroll
-            for (int i = 0; i < NL + 2; i++) {
+            for (int i = 0; i <= k + 2; i++) {
                 uint32_t co;
                 R[i] = addc(R[i], Sv[i], c, &co);
                 c = co;

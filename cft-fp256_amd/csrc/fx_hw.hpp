@@ -1,5 +1,5 @@
-// The two gfx950 instructions the limb arithmetic is written around, and the
-// HIP runtime header.  Included as <fx_hw.hpp> so that a portable CPU build of
+// The gfx950 instructions the limb arithmetic is written around, two binary64
+// estimates, and the HIP runtime header.  Included as <fx_hw.hpp> so that a portable CPU build of
 // the same arithmetic (the test suite's) can supply its own definitions.
 #pragma once
 
@@ -40,6 +40,35 @@ __device__ __forceinline__ uint32_t addc(uint32_t x, uint32_t y, uint32_t cin, u
 __device__ __forceinline__ uint32_t subc(uint32_t x, uint32_t y, uint32_t bin, uint32_t *bout)
 {
     return __builtin_subc(x, y, bin, bout);
+}
+
+// 1/x and sqrt(x), correctly rounded, for x between 2^62 and 2^66 (the digit
+// estimates' operands): the compiler's own sequences for 1.0 / x and sqrt(x)
+// - a hardware estimate, Newton steps, a final fused correction - without
+// their range scaling and special-case fix-ups, which leave a finite x of
+// this size unchanged.  The same bits as the division and the root.
+__device__ __forceinline__ double recip_mid(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(e, y, y);
+}
+
+__device__ __forceinline__ double sqrt_mid(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, s, 0.5);
+    s = __builtin_fma(s, r, s);
+    double d = __builtin_fma(-s, s, x);
+    h = __builtin_fma(h, r, h);
+    s = __builtin_fma(d, h, s);
+    d = __builtin_fma(-s, s, x);
+    return __builtin_fma(d, h, s);
 }
 
 } // namespace fx

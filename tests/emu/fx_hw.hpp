@@ -1,5 +1,5 @@
-// Portable stand-ins for the two gfx950 instructions of the device limb
-// arithmetic (cft-fp256_amd/csrc/fx_hw.hpp), so that the same fx_*.hpp
+// Portable stand-ins for the gfx950 instructions and binary64 estimates of the
+// device limb arithmetic (cft-fp256_amd/csrc/fx_hw.hpp), so that the same fx_*.hpp
 // headers compile with g++ and the test suite can check their logic against
 // the oracle without a GPU.  Test infrastructure only.
 #pragma once
@@ -40,5 +40,9 @@ inline uint32_t subc(uint32_t x, uint32_t y, uint32_t bin, uint32_t *bout)
     *bout = (uint32_t)(t >> 63);
     return (uint32_t)t;
 }
+
+// the device's recip_mid and sqrt_mid are correctly rounded for their range
+inline double recip_mid(double x) { return 1.0 / x; }
+inline double sqrt_mid(double x) { return sqrt(x); }
 
 } // namespace fx
