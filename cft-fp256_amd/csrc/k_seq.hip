@@ -834,7 +834,7 @@ __device__ __forceinline__ uint32_t instance_flags(uint32_t *s_lflags, uint32_t 
     return v;
 }
 
-template <int NL>
+template <int NL, bool RECS_LDS>
 __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide Wd,
                                                   int phys_a, int phys_b, int phys_c)
 {
@@ -848,7 +848,7 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
 
     // LDS: records (when they fit) | constants | register files | lane-flag words
     uint4 *s_recs = smem;
-    const uint32_t rec_rows = Wd.recs_in_lds ? Wd.nsteps * W : 0;
+    const uint32_t rec_rows = RECS_LDS ? Wd.nsteps * W : 0;
     RT *s_konst = reinterpret_cast<RT *>(s_recs + rec_rows);
     const uint32_t konst_rows = A.n_consts * NQ;
     RT *s_regs = s_konst + konst_rows;
@@ -869,7 +869,6 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
     WideInst<NL> I;
     I.recs_lds = (LdsRec)s_recs;
     I.recs_hbm = (HbmRec)Wd.recs;
-    const bool recs_lds = Wd.recs_in_lds != 0;
     I.nsteps = Wd.nsteps;
     I.W = W;
     I.j = j;
@@ -902,8 +901,7 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
         const uint32_t left = valid ? B.steps[id] : 0u;
         tile_priority(B.hi_steps, left);
         uint32_t *st = B.state + (size_t)id * A.n_in * NL;
-        const uint32_t lfl = recs_lds ? wide_run<NL, true>(I, valid, left, st, src, phys, id)
-                                      : wide_run<NL, false>(I, valid, left, st, src, phys, id);
+        const uint32_t lfl = wide_run<NL, RECS_LDS>(I, valid, left, st, src, phys, id);
         const uint32_t inst = instance_flags(s_lflags, g, j, lfl);
         if (j == 0 && valid && B.lflags)
             B.lflags[id] = (uint8_t)inst;
@@ -1204,18 +1202,30 @@ hipError_t launch_lanes(void *regs, void *pool, uint64_t stride, uint64_t n, uin
     return hipGetLastError();
 }
 
+// The step records in LDS or read from device memory: one kernel each, so
+// that each holds one copy of the step loop (a kernel with both ran up to
+// 1.5x slower, or not, by where the loader placed its code).
+template <int NL, bool RECS_LDS>
+hipError_t launch_wide_as(const SeqArgs &A, const SeqBatch &B, const SeqWide &Wd, int pa, int pb,
+                          int pc, int waves, int blocks, size_t lds, hipStream_t st)
+{
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_wide<NL, RECS_LDS>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_seq_wide<NL, RECS_LDS>), dim3((uint32_t)blocks), dim3(waves * kWave),
+                       lds, st, A, B, Wd, pa, pb, pc);
+    return hipGetLastError();
+}
+
 template <int NL>
 hipError_t launch_wide(const SeqArgs &A, const SeqBatch &B, const SeqWide &Wd, int pa, int pb,
                        int pc, int waves, int blocks, size_t lds, hipStream_t st)
 {
-    if (lds > 65536) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_wide<NL>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipGetLastError();
-    }
-    hipLaunchKernelGGL((k_seq_wide<NL>), dim3((uint32_t)blocks), dim3(waves * kWave), lds, st,
-                       A, B, Wd, pa, pb, pc);
-    return hipGetLastError();
+    return Wd.recs_in_lds
+               ? launch_wide_as<NL, true>(A, B, Wd, pa, pb, pc, waves, blocks, lds, st)
+               : launch_wide_as<NL, false>(A, B, Wd, pa, pb, pc, waves, blocks, lds, st);
 }
 
 template <int NL>
@@ -1249,21 +1259,28 @@ int multi_occupancy(int waves, size_t lds)
     return nb > 0 ? nb : 1;
 }
 
-template <int NL>
-int wide_occupancy(int waves, size_t lds)
+template <int NL, bool RECS_LDS>
+int wide_occupancy_as(int waves, size_t lds)
 {
     int nb = 0;
     if (lds > 65536) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_wide<NL>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_seq_wide<NL, RECS_LDS>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         (void)hipGetLastError();
     }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_seq_wide<NL>, waves * kWave, lds) !=
-        hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_seq_wide<NL, RECS_LDS>, waves * kWave,
+                                                     lds) != hipSuccess) {
         (void)hipGetLastError();
         return 1;
     }
     return nb > 0 ? nb : 1;
+}
+
+template <int NL>
+int wide_occupancy(int waves, size_t lds, bool recs_lds)
+{
+    return recs_lds ? wide_occupancy_as<NL, true>(waves, lds)
+                    : wide_occupancy_as<NL, false>(waves, lds);
 }
 
 } // namespace
@@ -1383,11 +1400,12 @@ extern "C" int cftk_seq_wide_blocks_per_cu(int fmt, uint32_t n_consts, uint32_t 
                                            uint32_t width, uint32_t nvregs, int waves)
 {
     const size_t lds = cftk_seq_wide_lds(fmt, n_consts, nsteps_in_lds, width, nvregs, waves);
+    const bool recs = nsteps_in_lds != 0;
     switch (fmt) {
-    case 0: return wide_occupancy<1>(waves, lds);
-    case 1: return wide_occupancy<2>(waves, lds);
-    case 2: return wide_occupancy<4>(waves, lds);
-    default: return wide_occupancy<8>(waves, lds);
+    case 0: return wide_occupancy<1>(waves, lds, recs);
+    case 1: return wide_occupancy<2>(waves, lds, recs);
+    case 2: return wide_occupancy<4>(waves, lds, recs);
+    default: return wide_occupancy<8>(waves, lds, recs);
     }
 }
 
