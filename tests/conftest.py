@@ -77,9 +77,13 @@ def emu():
             if f.endswith(".hpp")] + [os.path.join(EMU_DIR, "fx_hw.hpp"), src]
     if not os.path.exists(EMU_SO) or any(os.path.getmtime(h) > os.path.getmtime(EMU_SO)
                                          for h in hdrs):
+        # built under a name of this process's own, then moved into place in
+        # one step: parallel test workers may rebuild it at the same time
+        tmp = f"{EMU_SO}.{os.getpid()}"
         subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wno-unknown-pragmas",
-                        "-I", EMU_DIR, "-I", os.path.join(PKG, "csrc"), src, "-o", EMU_SO],
+                        "-I", EMU_DIR, "-I", os.path.join(PKG, "csrc"), src, "-o", tmp],
                        check=True)
+        os.replace(tmp, EMU_SO)
     e = ctypes.CDLL(EMU_SO)
     e.emu_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V, V, V,
                           ctypes.c_size_t, U32P]
