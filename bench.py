@@ -67,7 +67,7 @@ FMA_BYTES = 128          # 3 x 32 B in, 32 B out per element
 # HBM bytes an FMA element actually moves: rocprofv3 FETCH_SIZE (x2.000) +
 # WRITE_SIZE (x1.000) at 2^22 elements, each scaled by the same counter on a
 # plain 256 MiB copy (profiles/fma_fp256_elementwise.md)
-FMA_TRAFFIC_PER_ELEM = (393543.0 + 131129.0) * 1024 / 4194304
+FMA_TRAFFIC_PER_ELEM = (393510.0 + 131151.0) * 1024 / 4194304
 BYTES = {"add": 96, "mul": 96, "fma": 128, "div": 96, "sqrt": 64}
 # VALU: 256 CUs x 4 SIMDs x 2.4 GHz.  gfx950 issues a plain 32-bit wave64
 # operation every 2 cycles (32 lanes a cycle: the 157.3 TFLOP/s vector fp32
@@ -80,8 +80,8 @@ VALU_4CYCLE_WAVE_INSNS = 256 * 4 * 2.4e9 / 4
 # cycles a SIMD spends a VALU instruction there, GRBM_GUI_ACTIVE / 8 XCDs
 # over SQ_INSTS_VALU / 1,024 SIMDs: profiles/valu_issue_rates.md); the exact
 # product's 32x32->64 multiply-accumulates an element (8 x 8 limbs)
-FMA_VALU_PER_ELEM = 676.7
-FMA_CYCLES_PER_VALU = 5.17
+FMA_VALU_PER_ELEM = 654.8
+FMA_CYCLES_PER_VALU = 5.49
 FMA_MACS = 64
 
 
