@@ -520,6 +520,26 @@ __device__ __forceinline__ void tile_priority(uint32_t hi_steps, uint32_t left)
         __builtin_amdgcn_s_setprio(0);
 }
 
+// A persistent wave's tiles: its first by its place in the grid, wave-major
+// (wave w of every workgroup before wave w + 1 of any), the rest from the
+// queue, after those.  Tiles come longest first; handed out by the queue
+// alone, the first ones went to whichever waves reached it first, which
+// could put several of the longest on one CU and SIMD, where they split its
+// issue at the same priority.  Workgroups are spread over the CUs in order,
+// so wave-major order spreads the longest tiles too.
+__device__ __forceinline__ uint32_t first_tile(uint32_t wave)
+{
+    return uni(wave * gridDim.x + blockIdx.x);
+}
+
+__device__ __forceinline__ uint32_t later_tile(uint32_t *queue, uint32_t waves)
+{
+    uint32_t t = 0;
+    if (__lane_id() == 0)
+        t = atomicAdd(queue, 1u);
+    return gridDim.x * waves + uni(__shfl(t, 0));
+}
+
 template <int NL>
 __global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32_t nregs,
                                                    int phys_a, int phys_b, int phys_c,
@@ -536,6 +556,8 @@ __global__ __launch_bounds__(256) void k_seq_batch(SeqArgs A, SeqBatch B, uint32
     const int phys[3] = {phys_a, phys_b, phys_c};
     uint32_t fl = 0, status = 0;
 
+    // tiles from the queue alone: the first-tile order (first_tile) takes
+    // this kernel over 128 VGPRs (4 -> 3 waves a SIMD)
     for (;;) {
         uint32_t tile = 0;
         if (lane == 0)
@@ -771,16 +793,21 @@ __device__ __forceinline__ uint32_t wide_run(const WideInst<NL> &I, const bool v
                         // the lanes of a bundle mix FMA, ADD, SUB and MUL: steer each
                         // to the FMA it is defined as (host/src/softfloat.c sf_steer:
                         // ADD a*1+c, SUB a*1+(-c), MUL a*b+(+-0, the product's sign))
-                        // so all of them run one path
-                        if (op == OP_ADD || op == OP_SUB) {
-                            enc_one<NL>(xb);
-                            if (op == OP_SUB)
-                                xc[NL - 1] ^= 0x80000000u;
-                        } else if (op == OP_MUL) {
-                            const uint32_t sg = (xa[NL - 1] ^ xb[NL - 1]) & 0x80000000u;
-                            zero<NL>(xc);
-                            xc[NL - 1] = sg;
-                        }
+                        // so all of them run one path.  By selects, not branches: the
+                        // lanes of a bundle take different ones, and the branches
+                        // cost register moves under every exec mask
+                        const bool mul = op == OP_MUL;
+                        const uint32_t sg = (xa[NL - 1] ^ xb[NL - 1]) & 0x80000000u;
+                        uint32_t one[NL];
+                        enc_one<NL>(one);
+#pragma unroll
+                        for (int k = 0; k < NL; k++)
+                            xb[k] = addsub ? one[k] : xb[k];
+#pragma unroll
+                        for (int k = 0; k < NL - 1; k++)
+                            xc[k] = mul ? 0u : xc[k];
+                        xc[NL - 1] = mul ? sg
+                                         : xc[NL - 1] ^ (op == OP_SUB ? 0x80000000u : 0u);
                         fma_core<NL>(OP_FMA, xa, xb, xc, (int)WR_RND(rec.x), o, f);
                     } else {
                         misc_op<NL>((int)op, xa, xb, xc, o, f);
@@ -888,13 +915,7 @@ __global__ __launch_bounds__(256) void k_seq_wide(SeqArgs A, SeqBatch B, SeqWide
     const int phys[3] = {phys_a, phys_b, phys_c};
     uint32_t fl = 0, status = 0;
 
-    for (;;) {
-        uint32_t tile = 0;
-        if (lane == 0)
-            tile = atomicAdd(B.queue, 1u);
-        tile = uni(__shfl(tile, 0));
-        if (tile >= B.ntiles)
-            break;
+    for (uint32_t tile = first_tile(wave); tile < B.ntiles; tile = later_tile(B.queue, waves)) {
         const uint64_t idx = (uint64_t)tile * ipw + g;
         const bool valid = idx < B.n;
         const uint32_t id = valid ? B.order[idx] : 0u;
@@ -935,13 +956,7 @@ __global__ __launch_bounds__(256) void k_seq_multi(SeqArgs A, SeqMulti M)
     const int phys[3] = {-1, -1, -1};
     uint32_t fl = 0, status = 0, staged = 0xFFFFFFFFu;
 
-    for (;;) {
-        uint32_t tile = 0;
-        if (lane == 0)
-            tile = atomicAdd(M.queue, 1u);
-        tile = uni(__shfl(tile, 0));
-        if (tile >= M.ntiles)
-            break;
+    for (uint32_t tile = first_tile(wave); tile < M.ntiles; tile = later_tile(M.queue, waves)) {
         const uint4 T = M.tiles[tile];
         const uint32_t pi = uni(T.x), first = uni(T.y), count = uni(T.z);
         const SeqMProg *P = M.progs + pi;
