@@ -69,16 +69,21 @@ __device__ __forceinline__ void mul_limbs(const uint32_t (&x)[NA], const uint32_
     uint32_t hi = 0;
 #pragma unroll
     for (int k = 0; k < NA + NB - 1; k++) {
+        // a column's first product sets hi to its carry (the last column's
+        // hi went into acc's top half below)
+        const int i0 = k < NB ? 0 : k - NB + 1;
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int j = k - i;
             if (j < 0 || j >= NB)
                 continue;
-            mac(acc, hi, x[i], y[j]);
+            if (i == i0)
+                mac_first(acc, hi, x[i], y[j]);
+            else
+                mac(acc, hi, x[i], y[j]);
         }
         p[k] = (uint32_t)acc;
         acc = (acc >> 32) | ((uint64_t)hi << 32);
-        hi = 0;
     }
     p[NA + NB - 1] = (uint32_t)acc;
 }

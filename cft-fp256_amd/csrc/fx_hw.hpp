@@ -30,6 +30,15 @@ __device__ __forceinline__ void mac(uint64_t &acc, uint32_t &hi, uint32_t a, uin
     asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(hi), "=s"(cc) : "s"(cc));
 }
 
+// The same for the first product of a column: hi = the carry out, not hi +
+// it, so a column needs no instruction to clear hi first.
+__device__ __forceinline__ void mac_first(uint64_t &acc, uint32_t &hi, uint32_t a, uint32_t b)
+{
+    uint64_t cc;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc), "=s"(cc) : "v"(a), "v"(b));
+    asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(hi) : "s"(cc));
+}
+
 // x + y + cin (cin 0 or 1), carry out: one v_add_co / v_addc in a chain.
 __device__ __forceinline__ uint32_t addc(uint32_t x, uint32_t y, uint32_t cin, uint32_t *cout)
 {

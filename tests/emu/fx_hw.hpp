@@ -27,6 +27,14 @@ inline void mac(uint64_t &acc, uint32_t &hi, uint32_t a, uint32_t b)
     acc = s;
 }
 
+inline void mac_first(uint64_t &acc, uint32_t &hi, uint32_t a, uint32_t b)
+{
+    const uint64_t p = (uint64_t)a * b;
+    const uint64_t s = acc + p;
+    hi = s < p ? 1u : 0u;
+    acc = s;
+}
+
 inline uint32_t addc(uint32_t x, uint32_t y, uint32_t cin, uint32_t *cout)
 {
     const uint64_t s = (uint64_t)x + y + cin;
