@@ -80,8 +80,8 @@ VALU_4CYCLE_WAVE_INSNS = 256 * 4 * 2.4e9 / 4
 # cycles a SIMD spends a VALU instruction there, GRBM_GUI_ACTIVE / 8 XCDs
 # over SQ_INSTS_VALU / 1,024 SIMDs: profiles/valu_issue_rates.md); the exact
 # product's 32x32->64 multiply-accumulates an element (8 x 8 limbs)
-FMA_VALU_PER_ELEM = 654.8
-FMA_CYCLES_PER_VALU = 5.49
+FMA_VALU_PER_ELEM = 640.8
+FMA_CYCLES_PER_VALU = 5.65
 FMA_MACS = 64
 
 
